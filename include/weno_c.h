@@ -1,0 +1,145 @@
+/*
+ * libweno_mi355 -- C ABI of the MI355X (gfx950) WENO reconstruction path.
+ *
+ * This is the boundary a Fortran host binds with iso_c_binding (see
+ * wenostandalone_amd/fortran/m_weno_standalone.f90) and Python binds with
+ * ctypes.  Plain C types only: pointers, sizes, ints and doubles.
+ *
+ * The reference has no C boundary: its "API" is the Fortran module
+ * m_weno_standalone of inputs/weno_standalone.f90, whose procedures run
+ * `!$omp target` regions on arrays mapped with `target enter data`.  Each
+ * entry point below names the reference procedure it replaces.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative code on error;
+ *     weno_last_error() gives the message.  Nothing is launched after a
+ *     validation failure.
+ *   - Array arguments are device addresses, or host addresses that are
+ *     present in the data environment (weno_target_enter_data), in which case
+ *     the mapped device copy is used -- the semantics of OpenMP's implicit
+ *     map of a present array.  A host address that is not present is an
+ *     error (never dereferenced on the device).
+ *   - Calls are synchronous (like the reference's target regions, which
+ *     carry no `nowait`) unless weno_set_async(1) was called; then they are
+ *     stream-ordered on the library stream and weno_sync() waits.
+ *   - Layouts are the reference's column-major arrays:
+ *       v_vf       (nx+6, ny+6, nz+6, v_size)    weno_standalone.f90:163-165
+ *       v_rs_ws_d  (ns+6, nk, nl, v_size)        :167-169, :281-330
+ *       vL_d, vR_d (ns, nk, nl, v_size)          :343, :363, :383
+ *     with nx = m_cells+1+2*buff_size (likewise ny, nz) and, per direction,
+ *     (ns,nk,nl) = x:(nx,ny,nz)  y:(ny,nx,nz)  z:(nz,ny,nx).
+ *     Coefficients (nc = ns-6 core points):
+ *       polyL/polyR (nc,5,3), dL/dR (5,nc), beta (nc,5,6)   :171-187
+ */
+#ifndef WENO_C_H
+#define WENO_C_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WENO_OK 0
+#define WENO_E_INVALID (-1)   /* argument validation failed */
+#define WENO_E_HIP (-2)       /* HIP runtime error */
+#define WENO_E_NOT_PRESENT (-3) /* host address not in the data environment */
+#define WENO_E_STATE (-4)     /* library not initialised / wrong state */
+
+enum { WENO_DIR_X = 0, WENO_DIR_Y = 1, WENO_DIR_Z = 2 };
+enum { WENO_SRC_FIELD = 0, WENO_SRC_RESHAPED = 1 };
+enum { WENO_F64 = 0, WENO_F32 = 1, WENO_BF16 = 2 };
+
+/* reconstruct flags */
+#define WENO_FUSE_MP 1     /* apply s_preserve_monotonicity in the same pass */
+#define WENO_FUSE_COPY 2   /* write order-1 values (s_order1_copy) into j outside the core */
+#define WENO_STREAMING 4   /* non-temporal stores */
+
+/* Grid and scheme parameters: namelist &grid_params (weno_standalone.f90:
+ * 57-66, :117-118) plus `teno`, a compile-time parameter in the reference
+ * (:74) and a runtime switch here.  wenojs takes precedence over teno
+ * (:552, :1475); with neither, WENO outputs are NaN exactly as in the
+ * reference (no weight branch taken). */
+typedef struct weno_grid {
+    int m_cells, n_cells, p_cells;
+    int v_size;
+    int buff_size;
+    double weno_eps;
+    int wenojs;
+    int teno;
+    double teno_CT;
+} weno_grid;
+
+/* One direction's coefficient set (device or present host addresses). */
+typedef struct weno_coeffs {
+    const double* polyL;
+    const double* polyR;
+    const double* dL;
+    const double* dR;
+    const double* beta;
+} weno_coeffs;
+
+/* ---- runtime ------------------------------------------------------------ */
+/* Selects the device and creates the library stream.  Replaces the implicit
+ * OpenMP offload runtime start-up (omp_get_num_devices, :1755-1756). */
+int weno_init(int device);
+int weno_finalize(void);
+const char* weno_last_error(void);
+int weno_sync(void);
+int weno_set_async(int async_mode);
+/* Use a caller's hipStream_t (NULL = library stream). */
+int weno_set_stream(void* hip_stream);
+void* weno_get_stream(void);
+int weno_device_count(void);
+
+/* ---- data environment: `!$omp target enter/update/exit data` ------------
+ * :194-203, :209-212, :1790-1791, :1831, :1856 */
+int weno_target_enter_data(const void* host, size_t bytes); /* map(alloc:) */
+int weno_target_update_to(const void* host, size_t bytes);
+int weno_target_update_from(void* host, size_t bytes);
+int weno_target_exit_data(const void* host);                /* map(release:) */
+void* weno_device_address(const void* host);                /* NULL if absent */
+
+/* plain device memory for callers without an allocator of their own */
+void* weno_device_alloc(size_t bytes);
+int weno_device_free(void* dev);
+int weno_memcpy_to_device(void* dev, const void* host, size_t bytes);
+int weno_memcpy_to_host(void* host, const void* dev, size_t bytes);
+
+/* ---- shapes ------------------------------------------------------------- */
+size_t weno_field_elems(const weno_grid* g);             /* v_vf          */
+size_t weno_reshaped_elems(const weno_grid* g, int dir); /* v_rs_ws_dir   */
+size_t weno_face_elems(const weno_grid* g, int dir);     /* vL_dir, vR_dir */
+int weno_core_count(const weno_grid* g, int dir);        /* nc            */
+
+/* ---- kernels ----------------------------------------------------------- */
+/* s_initialize_weno_{x,y,z} (:281-330): v_rs_ws_dir = permuted v_vf.
+ * dtype WENO_F64 or WENO_F32. */
+int weno_reshape(const weno_grid* g, int dir, int dtype, const void* v_vf, void* v_rs);
+
+/* s_order1_copy_{x,y,z} (:332-390): vL = vR = v_vf in reshaped coordinates.
+ * dtype of the field; F64 -> double faces, F32 and BF16 -> float faces. */
+int weno_order1_copy(const weno_grid* g, int dir, int dtype, const void* v_vf, void* vL, void* vR);
+
+/* s_weno3_kernel (:392), s_weno5_kernel / s_weno5_kernel_dispatch
+ * (:477, :1399), s_weno5_kernel_{x,y,z} (:671, :829, :987),
+ * s_weno5_{left,right}_kernel (:1145, :1272 -- same outputs), s_weno7_kernel
+ * (:1419).  order in {3,5,7}.  src_kind WENO_SRC_RESHAPED takes v_rs_ws_dir
+ * as the reference does; WENO_SRC_FIELD reads v_vf directly with
+ * direction-aware loads (no reshape pass).  Writes the core range
+ * (order 5/7) or the whole `is` range (order 3); with WENO_FUSE_COPY the
+ * rest of vL/vR gets the order-1 values; with WENO_FUSE_MP the result is
+ * additionally limited by s_preserve_monotonicity.
+ * dtype: WENO_F64 (double field, double faces), WENO_F32 (float field,
+ * float faces), WENO_BF16 (bf16 field, float faces); FP32/BF16 do the
+ * stencil arithmetic in FP32 and beta and the weight normalisation in FP64. */
+int weno_reconstruct(const weno_grid* g, int order, int dir, int src_kind, int dtype,
+                     const void* src, const weno_coeffs* c, void* vL, void* vR, int flags);
+
+/* s_preserve_monotonicity (:1544-1615): limits vL/vR in place on the core. */
+int weno_mp_limit(const weno_grid* g, int dir, int src_kind, int dtype, const void* src,
+                  void* vL, void* vR);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

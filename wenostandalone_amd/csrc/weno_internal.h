@@ -1,0 +1,64 @@
+// Internal declarations shared by the HIP translation units of libweno_mi355.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+
+namespace wm {
+
+enum Op { OP_RESHAPE = 0, OP_COPY = 1, OP_WENO = 2, OP_MP = 3 };
+enum Mode { MODE_NONE = 0, MODE_JS = 1, MODE_TENO = 2 };
+enum Dtype { DT_F64 = 0, DT_F32 = 1, DT_BF16 = 2 };
+
+// Per-direction stencil coefficients, in the reference's layouts
+// (weno_standalone.f90:171-187): poly (nc,5,3), d (5,nc), beta (nc,5,6).
+struct Coefs {
+    const double* polyL;
+    const double* polyR;
+    const double* dL;
+    const double* dR;
+    const double* beta;
+    int nc;
+};
+
+// One sweep over a set of pencils.  A pencil is one line along the sweep
+// direction at fixed transverse position (a,b) and variable i.  (a,b) are
+// the transverse coordinates in load order: `a` is the one whose input
+// stride is smallest, so that consecutive pencils of a tile sit next to each
+// other in memory.
+struct TileArgs {
+    const void* src;
+    long long src_off;            // element offset of (J=0, a=0, b=0, i=0)
+    long long s_j, s_a, s_b, s_i; // input strides (elements)
+    int src_len;                  // input extent along the sweep (ns+6)
+    int na, nb, nv;
+    int a_is_k;                   // pencil (a,b) -> output (k,l) = a_is_k ? (a,b) : (b,a)
+    void* outL;
+    void* outR;
+    long long o_k, o_l, o_i;      // output strides; stride along j is 1
+    int out_len;                  // output row length (ns; ns+6 for reshape)
+    int J0f;                      // input J of output j (J = j + J0f)
+    int j_beg, j_end;             // reconstructed (core) range, inclusive; empty if j_beg>j_end
+    int copy_halo;                // write order-1 values into the rest of [0,out_len)
+    int chunk;                    // outputs per tile along j
+    int chunks;                   // tiles along j
+    int tiles_per_var;            // pencil groups per variable
+    int ntiles;
+    int ldj;                      // LDS row stride (elements)
+    Coefs c;
+    double eps, teno_ct;
+};
+
+struct Launch {
+    int op, order, mode, mp, dtype;
+    int np;          // pencils per tile
+    int nontemporal; // streaming stores
+};
+
+// Launches one tile-kernel sweep; returns hipSuccess or an error.
+hipError_t launch_tile(const TileArgs& a, const Launch& l, hipStream_t st);
+
+// LDS element size of the compute type for a storage dtype
+inline int lds_elem_bytes(int dtype) { return dtype == DT_F64 ? 8 : 4; }
+
+}  // namespace wm
