@@ -1,0 +1,223 @@
+#!/usr/bin/env python3
+"""Benchmark: fused FP64 WENO5-JS reconstruction, 200^3 x 8, one MI355X.
+
+One step = the reference's WENO5 reconstruction in `kernel` scope
+(s_run_weno5_reconstruction with the specialised x/y/z kernels,
+inputs/weno_standalone.f90:1713-1724): three sweeps (x, y, z), each writing
+the core range of vL/vR (2*206*212*212*8 = 148,135,424 face values per sweep)
+straight from v_vf with direction-aware loads.  Inputs are resident in HBM
+before the timed region.  value = face values reconstructed per second.
+
+Prints one JSON line.  Also reports the dominant kernel's roofline figure
+(algorithmic bytes per launch / HIP-event launch time on the launch stream)
+and a CPU baseline (the reference's own OpenMP build on the host cores when
+present, else the oracle's C port).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
+
+
+def hip():
+    h = C.CDLL("libamdhip64.so")
+    h.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+    h.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+    h.hipEventSynchronize.argtypes = [C.c_void_p]
+    h.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+    h.hipEventDestroy.argtypes = [C.c_void_p]
+    h.hipDeviceSynchronize.argtypes = []
+    return h
+
+
+def cpu_baseline(m: int, v: int, buff: int, budget_s: float = 30.0):
+    """The reference's own CPU OpenMP build (oracle/_ref/weno_standalone,
+    compiled from the reference source) in weno5/kernel scope on the host
+    cores; falls back to the oracle's C port."""
+    cores = max(1, min(16, os.cpu_count() or 1))
+    faces = 3 * 2 * (m + 1 + 2 * buff - 6) * (m + 1 + 2 * buff) ** 2 * v
+    ref = os.path.join(ROOT, "oracle", "_ref", "weno_standalone")
+    if os.path.exists(ref):
+        with tempfile.TemporaryDirectory() as td:
+            nml = os.path.join(td, "in.nml")
+            with open(nml, "w") as fh:
+                fh.write(f"&grid_params\n m_cells={m}, n_cells={m}, p_cells={m}, v_size={v},\n"
+                         f" buff_size={buff}, weno_eps=1e-16, wenojs=T, teno_CT=1e-5\n/\n")
+            env = dict(os.environ, OMP_TARGET_OFFLOAD="DISABLED", OMP_NUM_THREADS=str(cores),
+                       OMP_PROC_BIND="close", WENO_KERNEL_MODE="weno5", WENO5_BENCH_SCOPE="kernel",
+                       WENO5_SPECIALIZED_COMBINED="1", WENO_WARMUP_ITERS="1", WENO_BENCH_ITERS="3")
+            out = subprocess.run([ref, nml], env=env, capture_output=True, text=True,
+                                 timeout=budget_s * 4).stdout
+        t = float(re.search(r"Kernel time per iter\s*:\s*([0-9.Ee+-]+)", out).group(1))
+        cpu = ""
+        try:
+            cpu = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
+        except Exception:
+            pass
+        return {"value": faces / t, "unit": "face values/s", "cores": cores, "kind": "reference",
+                "sample": f"reference OpenMP build, weno5 kernel scope (x+y+z), {m + 1}^3x{v}, "
+                          f"1 warmup + 3 timed iterations, {t:.4f} s/iter; host {cpu}"}
+    import oracle
+    from wenostandalone_amd import fields
+    mm = 99   # bounded sample
+    g = oracle.grid(mm, v=v, buff=buff)
+    f = fields.seeded_field(mm, mm, mm, v, buff, seed=0)
+    t0 = time.perf_counter()
+    n = 0
+    for d in range(3):
+        ns = oracle.dir_extents(g, d)[0]
+        cs = oracle.fill_coefficients(ns - 6, 0.03)
+        rs = oracle.reshape(g, d, f)
+        vL, vR = oracle.order1_copy(g, d, f)
+        t1 = time.perf_counter()
+        oracle.weno(g, 5, d, rs, cs, vL, vR)
+        n += time.perf_counter() - t1
+    sfaces = 3 * 2 * (mm + 1 + 2 * buff - 6) * (mm + 1 + 2 * buff) ** 2 * v
+    return {"value": sfaces / n, "unit": "face values/s", "cores": cores, "kind": "port",
+            "sample": f"oracle C port, WENO5 x+y+z at {mm + 1}^3x{v} ({n:.3f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--m", type=int, default=199)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--flags", type=int, default=0, help="extra weno_reconstruct flags")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the sweep would shard "
+              "by sub-domain with a buff_size halo exchange; see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import wenostandalone_amd as wm
+    from wenostandalone_amd import fields
+    from wenostandalone_amd.device import DeviceArray
+
+    m, v, buff = args.m, 8, 6
+    wm.check(wm.lib.weno_init(0))
+    g = wm.make_grid(m, v=v, buff=buff)
+    f = fields.seeded_field(m, m, m, v, buff, seed=0)
+    vf = DeviceArray.from_host(f)
+    del f
+    nx = m + 1 + 2 * buff
+    ncore = nx - 6
+    coefs, faces = [], []
+    for d in range(3):
+        devs = [DeviceArray.from_host(c) for c in fields.harness_coefficients(ncore, fields.DIR_OFFSETS[d])]
+        coefs.append((devs, wm.Coeffs(*[x.ptr for x in devs])))
+        L = DeviceArray((nx, nx, nx, v))
+        R = DeviceArray((nx, nx, nx, v))
+        wm.check(wm.lib.weno_order1_copy(C.byref(g), d, wm.F64, vf.vp, L.vp, R.vp))
+        faces.append((L, R))
+
+    wm.check(wm.lib.weno_set_async(1))
+    stream = wm.lib.weno_get_stream()
+    h = hip()
+
+    def step(evs=None):
+        for d in range(3):
+            if evs is not None:
+                h.hipEventRecord(evs[d][0], stream)
+            wm.check(wm.lib.weno_reconstruct(C.byref(g), 5, d, wm.SRC_FIELD, wm.F64, vf.vp,
+                                             C.byref(coefs[d][1]), faces[d][0].vp, faces[d][1].vp,
+                                             args.flags))
+            if evs is not None:
+                h.hipEventRecord(evs[d][1], stream)
+
+    for _ in range(args.warmup):
+        step()
+    wm.check(wm.lib.weno_sync())
+
+    evs = []
+    for _ in range(args.steps):
+        e = []
+        for d in range(3):
+            a, b = C.c_void_p(), C.c_void_p()
+            h.hipEventCreate(C.byref(a))
+            h.hipEventCreate(C.byref(b))
+            e.append((a, b))
+        evs.append(e)
+    t_begin, t_end = C.c_void_p(), C.c_void_p()
+    h.hipEventCreate(C.byref(t_begin))
+    h.hipEventCreate(C.byref(t_end))
+
+    h.hipDeviceSynchronize()
+    w0 = time.perf_counter()
+    h.hipEventRecord(t_begin, stream)
+    for s in range(args.steps):
+        step(evs[s])
+    h.hipEventRecord(t_end, stream)
+    wm.check(wm.lib.weno_sync())
+    h.hipDeviceSynchronize()
+    wall = time.perf_counter() - w0
+    ms = C.c_float()
+    h.hipEventElapsedTime(C.byref(ms), t_begin, t_end)
+    total_ms = ms.value
+    per_dir = [[], [], []]
+    for e in evs:
+        for d in range(3):
+            h.hipEventElapsedTime(C.byref(ms), e[d][0], e[d][1])
+            per_dir[d].append(ms.value)
+    wm.check(wm.lib.weno_set_async(0))
+
+    faces_per_sweep = 2 * ncore * nx * nx * v
+    ms_per_step = total_ms / args.steps
+    value = 3 * faces_per_sweep / (ms_per_step * 1e-3)
+    # algorithmic bytes per launch: distinct inputs (core +- 2 along the sweep)
+    # and the core-range outputs, FP64
+    bytes_launch = ((ncore + 4) * nx * nx * v + faces_per_sweep) * 8
+    launch_ms = float(np.mean([x for p in per_dir for x in p]))
+    achieved = bytes_launch / (launch_ms * 1e-3) / 1e9
+    out = {
+        "metric": "WENO5 cell-faces/s, 200^3x8 FP64 (% HBM roofline)",
+        "value": value,
+        "unit": "face values/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded field recipe 0: sine modes + step + noise; harness coefficients)",
+        "config": {"workload": f"WENO5-JS x+y+z sweeps from v_vf, {m + 1}^3 interior x {v} vars, "
+                               f"buff_size {buff}, core-range outputs",
+                   "m_cells": m, "v_size": v, "buff_size": buff, "parallelism": "single GPU"},
+        "per_sweep_ms": {k: float(np.median(p)) for k, p in zip("xyz", per_dir)},
+        "wall_s_timed_region": wall,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "sweep_kernel<OP_WENO,5,JS,noMP,double,double>",
+                     "bytes_per_launch": bytes_launch, "avg_launch_ms": launch_ms},
+    }
+    tr = os.environ.get("WENO_BENCH_TRAFFIC_BYTES")
+    if tr:
+        out["roofline"]["traffic"] = float(tr)
+    for L, R in faces:
+        L.free()
+        R.free()
+    vf.free()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(m, v, buff)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

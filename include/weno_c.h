@@ -53,6 +53,8 @@ enum { WENO_F64 = 0, WENO_F32 = 1, WENO_BF16 = 2 };
 #define WENO_FUSE_MP 1     /* apply s_preserve_monotonicity in the same pass */
 #define WENO_FUSE_COPY 2   /* write order-1 values (s_order1_copy) into j outside the core */
 #define WENO_STREAMING 4   /* non-temporal stores */
+#define WENO_ONLY_L 8      /* write vL only (s_weno5_left_kernel, :1145) */
+#define WENO_ONLY_R 16     /* write vR only (s_weno5_right_kernel, :1272) */
 
 /* Grid and scheme parameters: namelist &grid_params (weno_standalone.f90:
  * 57-66, :117-118) plus `teno`, a compile-time parameter in the reference

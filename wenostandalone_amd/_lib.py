@@ -16,7 +16,7 @@ WENO_OK = 0
 DIR_X, DIR_Y, DIR_Z = 0, 1, 2
 SRC_FIELD, SRC_RESHAPED = 0, 1
 F64, F32, BF16 = 0, 1, 2
-FUSE_MP, FUSE_COPY, STREAMING = 1, 2, 4
+FUSE_MP, FUSE_COPY, STREAMING, ONLY_L, ONLY_R = 1, 2, 4, 8, 16
 
 
 class WenoError(RuntimeError):

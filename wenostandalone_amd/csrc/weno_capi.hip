@@ -182,23 +182,56 @@ void output_map(const Dims& d, int row_len, TileArgs& a)
 void plan_tiles(TileArgs& a, Launch& l, int np_override, int chunk_override)
 {
     const int elem = lds_elem_bytes(l.dtype);
-    int np = np_override > 0 ? np_override : (l.dtype == DT_F64 ? 16 : 32);
-    if (np > 32) np = 32;
+    const int np_target = np_override > 0 ? np_override : (l.dtype == DT_F64 ? 16 : 32);
     const int halo = 8;                      // 2*R_max + 1, rounded up
-    const int max_lds = 48 * 1024;
-    int chunk = chunk_override > 0 ? chunk_override : a.out_len;
-    if (chunk > a.out_len) chunk = a.out_len;
-    if (static_cast<long long>(np) * (chunk + halo) * elem > max_lds) chunk = max_lds / (np * elem) - halo;
-    int chunks = (a.out_len + chunk - 1) / chunk;
+    const long long lds_budget = 96 * 1024;  // both tile buffers
+    const int max_np = kernel_max_np(), max_slots = kernel_max_slots();
+    int chunk = chunk_override > 0 ? std::min(chunk_override, a.out_len) : a.out_len;
+    const int min_np = std::min(np_target, 8);
+    if (2LL * min_np * (chunk + halo) * elem > lds_budget)
+        chunk = static_cast<int>(lds_budget / (2LL * min_np * elem)) - halo;
+    const int chunks = (a.out_len + chunk - 1) / chunk;
     chunk = (a.out_len + chunks - 1) / chunks;
-    int nJ = std::min(chunk + halo - 1, a.src_len);
-    a.ldj = nJ | 1;
+    // lanes per group: the most outputs any chunk row has
+    int jn = 1;
+    for (int c = 0; c < chunks; ++c) {
+        const int jo0 = c * chunk, jo1 = std::min(a.out_len, jo0 + chunk);
+        const int c0 = std::max(jo0, a.j_beg), c1 = std::min(jo1 - 1, a.j_end);
+        const int n = a.copy_halo ? jo1 - jo0 : std::max(0, c1 - c0 + 1);
+        jn = std::max(jn, n);
+    }
+    // lane groups per block: best lane utilisation within 1024 threads
+    int G = 1, T = 64 * ((jn + 63) / 64);
+    double best = double(jn) / T;
+    const int tmax = max_threads(l.op, l.order);
+    for (int g = 2; g <= 16; ++g) {
+        const int t = 64 * ((g * jn + 63) / 64);
+        if (t > tmax) break;
+        const double u = double(g * jn) / t;
+        if (u > best + 1e-9) {
+            best = u;
+            G = g;
+            T = t;
+        }
+    }
+    const int ldj = std::min(chunk + halo - 1, a.src_len) | 1;
+    int npg = std::max(1, (np_target + G - 1) / G);
+    npg = std::min(npg, std::max(1, max_np / G));
+    while (npg > 1 && (2LL * G * npg * ldj * elem > lds_budget ||
+                       (static_cast<long long>(G) * npg * ldj + T - 1) / T > max_slots))
+        --npg;
+    a.ldj = ldj;
     a.chunk = chunk;
     a.chunks = chunks;
-    l.np = np;
+    a.groups = G;
+    a.npg = npg;
+    a.np = G * npg;
+    a.jn = jn;
+    a.threads = T;
+    a.slots = static_cast<int>((static_cast<long long>(a.np) * ldj + T - 1) / T);
     const long long pen = static_cast<long long>(a.na) * a.nb;
-    a.tiles_per_var = static_cast<int>((pen + np - 1) / np);
-    a.ntiles = a.tiles_per_var * a.chunks * a.nv;
+    a.tiles_per_var = static_cast<int>((pen + a.np - 1) / a.np);
+    l.blocks_per_cu = 0;
 }
 
 int mode_for(const weno_grid* g, int order)
@@ -240,9 +273,7 @@ int resolve_coeffs(const weno_coeffs* c, Coefs& out, int nc)
     return WENO_OK;
 }
 
-int build_reconstruct(const weno_grid* g, int order, int dir, int src_kind, int dtype, const void* src,
-                      const weno_coeffs* c, void* vL, void* vR, int flags, TileArgs& a, Launch& l,
-                      int np_override, int chunk_override)
+int validate_reconstruct(const weno_grid* g, int order, int dir, int src_kind, int dtype, int flags)
 {
     int rc;
     if ((rc = check_grid(g)) || (rc = check_dir(dir)) || (rc = check_dtype(dtype))) return rc;
@@ -250,8 +281,20 @@ int build_reconstruct(const weno_grid* g, int order, int dir, int src_kind, int 
         return fail(WENO_E_INVALID, "order must be 3, 5 or 7 (got %d)", order);
     if (src_kind != WENO_SRC_FIELD && src_kind != WENO_SRC_RESHAPED)
         return fail(WENO_E_INVALID, "src_kind must be WENO_SRC_FIELD or WENO_SRC_RESHAPED");
-    if (flags & ~(WENO_FUSE_MP | WENO_FUSE_COPY | WENO_STREAMING))
+    if (flags & ~(WENO_FUSE_MP | WENO_FUSE_COPY | WENO_STREAMING | WENO_ONLY_L | WENO_ONLY_R))
         return fail(WENO_E_INVALID, "unknown flag bits 0x%x", flags);
+    if ((flags & WENO_ONLY_L) && (flags & WENO_ONLY_R))
+        return fail(WENO_E_INVALID, "WENO_ONLY_L and WENO_ONLY_R are exclusive (flag bits 0x%x)", flags);
+    return WENO_OK;
+}
+
+int build_reconstruct(const weno_grid* g, int order, int dir, int src_kind, int dtype, const void* src,
+                      const weno_coeffs* c, void* vL, void* vR, int flags, TileArgs& a, Launch& l,
+                      int np_override, int chunk_override)
+{
+    int rc;
+    if ((rc = validate_reconstruct(g, order, dir, src_kind, dtype, flags))) return rc;
+    if ((rc = ensure_init())) return rc;
     std::memset(&a, 0, sizeof a);
     const Dims d = dims_of(g, dir);
     const void* s;
@@ -273,7 +316,11 @@ int build_reconstruct(const weno_grid* g, int order, int dir, int src_kind, int 
         a.j_beg = 3;
         a.j_end = d.ns - 4;
     }
+    // s_preserve_monotonicity works on the core range only (:1562)
+    a.mp_beg = 3;
+    a.mp_end = d.ns - 4;
     a.copy_halo = (flags & WENO_FUSE_COPY) ? 1 : 0;
+    a.sides = (flags & WENO_ONLY_L) ? 1 : ((flags & WENO_ONLY_R) ? 2 : 3);
     a.eps = g->weno_eps;
     a.teno_ct = g->teno_CT;
     l.op = OP_WENO;
@@ -528,9 +575,10 @@ int weno_core_count(const weno_grid* g, int dir)
 int weno_reshape(const weno_grid* g, int dir, int dtype, const void* v_vf, void* v_rs)
 {
     int rc;
-    if ((rc = ensure_init()) || (rc = check_grid(g)) || (rc = check_dir(dir))) return rc;
+    if ((rc = check_grid(g)) || (rc = check_dir(dir))) return rc;
     if (dtype != WENO_F64 && dtype != WENO_F32)
         return fail(WENO_E_INVALID, "reshape supports WENO_F64 and WENO_F32 (got %d)", dtype);
+    if ((rc = ensure_init())) return rc;
     TileArgs a;
     std::memset(&a, 0, sizeof a);
     Launch l{};
@@ -539,7 +587,6 @@ int weno_reshape(const weno_grid* g, int dir, int dtype, const void* v_vf, void*
     void* o;
     if ((rc = resolve(v_vf, &s, "v_vf")) || (rc = resolve_mut(v_rs, &o, "v_rs_ws"))) return rc;
     input_map(d, dir, WENO_SRC_FIELD, a);
-    a.src_off -= 0;  // the reshaped array keeps the 3 padding cells along the sweep only
     output_map(d, d.ns + 6, a);
     a.src = s;
     a.outL = o;
@@ -548,6 +595,7 @@ int weno_reshape(const weno_grid* g, int dir, int dtype, const void* v_vf, void*
     a.j_beg = 1;
     a.j_end = 0;
     a.copy_halo = 1;
+    a.sides = 1;
     l.op = OP_RESHAPE;
     l.dtype = dtype;
     plan_tiles(a, l, 0, 0);
@@ -557,7 +605,7 @@ int weno_reshape(const weno_grid* g, int dir, int dtype, const void* v_vf, void*
 int weno_order1_copy(const weno_grid* g, int dir, int dtype, const void* v_vf, void* vL, void* vR)
 {
     int rc;
-    if ((rc = ensure_init()) || (rc = check_grid(g)) || (rc = check_dir(dir)) || (rc = check_dtype(dtype)))
+    if ((rc = check_grid(g)) || (rc = check_dir(dir)) || (rc = check_dtype(dtype)) || (rc = ensure_init()))
         return rc;
     TileArgs a;
     std::memset(&a, 0, sizeof a);
@@ -577,6 +625,7 @@ int weno_order1_copy(const weno_grid* g, int dir, int dtype, const void* v_vf, v
     a.j_beg = 1;
     a.j_end = 0;
     a.copy_halo = 1;
+    a.sides = 3;
     l.op = OP_COPY;
     l.dtype = dtype;
     plan_tiles(a, l, 0, 0);
@@ -587,7 +636,6 @@ int weno_reconstruct(const weno_grid* g, int order, int dir, int src_kind, int d
                      const weno_coeffs* c, void* vL, void* vR, int flags)
 {
     int rc;
-    if ((rc = ensure_init())) return rc;
     TileArgs a;
     Launch l{};
     if ((rc = build_reconstruct(g, order, dir, src_kind, dtype, src, c, vL, vR, flags, a, l,
@@ -599,10 +647,11 @@ int weno_reconstruct(const weno_grid* g, int order, int dir, int src_kind, int d
 int weno_mp_limit(const weno_grid* g, int dir, int src_kind, int dtype, const void* src, void* vL, void* vR)
 {
     int rc;
-    if ((rc = ensure_init()) || (rc = check_grid(g)) || (rc = check_dir(dir)) || (rc = check_dtype(dtype)))
+    if ((rc = check_grid(g)) || (rc = check_dir(dir)) || (rc = check_dtype(dtype)) || (rc = ensure_init()))
         return rc;
     if (src_kind != WENO_SRC_FIELD && src_kind != WENO_SRC_RESHAPED)
         return fail(WENO_E_INVALID, "src_kind must be WENO_SRC_FIELD or WENO_SRC_RESHAPED");
+    if ((rc = ensure_init())) return rc;
     TileArgs a;
     std::memset(&a, 0, sizeof a);
     Launch l{};
@@ -620,7 +669,10 @@ int weno_mp_limit(const weno_grid* g, int dir, int src_kind, int dtype, const vo
     a.J0f = 3;
     a.j_beg = 3;
     a.j_end = d.ns - 4;
+    a.mp_beg = 3;
+    a.mp_end = d.ns - 4;
     a.copy_halo = 0;
+    a.sides = 3;
     l.op = OP_MP;
     l.dtype = dtype;
     plan_tiles(a, l, 0, 0);

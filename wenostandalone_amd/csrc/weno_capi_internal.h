@@ -43,6 +43,7 @@ void input_map(const Dims& d, int dir, int src_kind, TileArgs& a);
 void output_map(const Dims& d, int row_len, TileArgs& a);
 void plan_tiles(TileArgs& a, Launch& l, int np_override, int chunk_override);
 int mode_for(const weno_grid* g, int order);
+int validate_reconstruct(const weno_grid* g, int order, int dir, int src_kind, int dtype, int flags);
 int build_reconstruct(const weno_grid* g, int order, int dir, int src_kind, int dtype, const void* src,
                       const weno_coeffs* c, void* vL, void* vR, int flags, TileArgs& a, Launch& l,
                       int np_override, int chunk_override);

@@ -40,10 +40,18 @@ struct TileArgs {
     int J0f;                      // input J of output j (J = j + J0f)
     int j_beg, j_end;             // reconstructed (core) range, inclusive; empty if j_beg>j_end
     int copy_halo;                // write order-1 values into the rest of [0,out_len)
+    int sides;                    // bit 0: write outL, bit 1: write outR
+    int mp_beg, mp_end;           // range the fused limiter applies to (inclusive)
     int chunk;                    // outputs per tile along j
     int chunks;                   // tiles along j
+    int np;                       // pencils per tile (= groups * npg)
+    int groups;                   // lane groups per block
+    int npg;                      // pencils per lane group
+    int jn;                       // lanes per group (outputs per chunk row)
+    int threads;                  // block size
+    int slots;                    // tile elements loaded per thread
     int tiles_per_var;            // pencil groups per variable
-    int ntiles;
+    int bpc;                      // blocks per chunk (set at launch)
     int ldj;                      // LDS row stride (elements)
     Coefs c;
     double eps, teno_ct;
@@ -51,9 +59,13 @@ struct TileArgs {
 
 struct Launch {
     int op, order, mode, mp, dtype;
-    int np;          // pencils per tile
-    int nontemporal; // streaming stores
+    int nontemporal;   // streaming stores
+    int blocks_per_cu; // 0 = occupancy limit
 };
+
+int kernel_max_slots();
+int kernel_max_np();
+int max_threads(int op, int order);
 
 // Launches one tile-kernel sweep; returns hipSuccess or an error.
 hipError_t launch_tile(const TileArgs& a, const Launch& l, hipStream_t st);

@@ -95,17 +95,14 @@ struct Cf;
 template <typename C>
 struct Cf<3, C> {
     C pl[2], pr[2];
-    double dl[2], dr[2], b[2];
 };
 template <typename C>
 struct Cf<5, C> {
     C pl[3][2], pr[3][2];
-    double dl[3], dr[3], b[3][3];
 };
 template <typename C>
 struct Cf<7, C> {
     C pl[4][3], pr[4][3];
-    double dl[4], dr[4], b[4][6];
 };
 
 template <typename C>
@@ -116,9 +113,6 @@ __device__ __forceinline__ void load_cf(const Coefs& c, int jc, Cf<3, C>& f)
     for (int r = 0; r < 2; ++r) {
         f.pl[r] = static_cast<C>(c.polyL[jc + nc * r]);
         f.pr[r] = static_cast<C>(c.polyR[jc + nc * r]);
-        f.b[r] = c.beta[jc + nc * r];
-        f.dl[r] = c.dL[r + 5 * jc];
-        f.dr[r] = c.dR[r + 5 * jc];
     }
 }
 template <typename C>
@@ -132,10 +126,6 @@ __device__ __forceinline__ void load_cf(const Coefs& c, int jc, Cf<5, C>& f)
             f.pl[r][s] = static_cast<C>(c.polyL[jc + nc * (r + 5 * s)]);
             f.pr[r][s] = static_cast<C>(c.polyR[jc + nc * (r + 5 * s)]);
         }
-#pragma unroll
-        for (int s = 0; s < 3; ++s) f.b[r][s] = c.beta[jc + nc * (r + 5 * s)];
-        f.dl[r] = c.dL[r + 5 * jc];
-        f.dr[r] = c.dR[r + 5 * jc];
     }
 }
 template <typename C>
@@ -149,12 +139,31 @@ __device__ __forceinline__ void load_cf(const Coefs& c, int jc, Cf<7, C>& f)
             f.pl[r][s] = static_cast<C>(c.polyL[jc + nc * (r + 5 * s)]);
             f.pr[r][s] = static_cast<C>(c.polyR[jc + nc * (r + 5 * s)]);
         }
-#pragma unroll
-        for (int s = 0; s < 6; ++s) f.b[r][s] = c.beta[jc + nc * (r + 5 * s)];
-        f.dl[r] = c.dL[r + 5 * jc];
-        f.dr[r] = c.dR[r + 5 * jc];
     }
 }
+
+// Smoothness-indicator coefficients beta_coef(j, r, s) and the linear weights
+// d_cbL/d_cbR(r, j) of one lane, kept in an LDS table [entry][lane] shared by
+// the block's lane groups (they hold the same j): entries r*NS+s for beta,
+// then NR entries of dL and NR of dR.  Reading them at use keeps them out of
+// the register budget.
+template <int NS>
+struct BetaRef {
+    const double* p;
+    int stride;
+    int nb;   // number of beta entries (NS*NR)
+    int nr;   // stencils
+    __device__ __forceinline__ double operator()(int r, int s) const { return p[(r * NS + s) * stride]; }
+    __device__ __forceinline__ double dl(int r) const { return p[(nb + r) * stride]; }
+    __device__ __forceinline__ double dr(int r) const { return p[(nb + nr + r) * stride]; }
+};
+template <int ORDER>
+struct BetaTraits {
+    static constexpr int NS = ORDER == 3 ? 1 : (ORDER == 5 ? 3 : 6);   // terms per stencil
+    static constexpr int NR = ORDER == 3 ? 2 : (ORDER == 5 ? 3 : 4);   // stencils
+    static constexpr int N = NS * NR;
+    static constexpr int ENTRIES = N + 2 * NR;
+};
 
 // P_r = prod_{s != r} q_s
 template <int N>
@@ -216,8 +225,8 @@ __device__ __forceinline__ double masked_combine(const double (&a)[N], const C (
 
 // ---- WENO3: v[0..2] = j-1, j, j+1 (weno_standalone.f90:392-475) ----------
 template <int MODE, typename C>
-__device__ __forceinline__ void weno3(const C* v, const Cf<3, C>& f, double eps, double,
-                                      double& oL, double& oR)
+__device__ __forceinline__ void weno3(const C* v, const Cf<3, C>& f, const BetaRef<1>& B, double eps,
+                                      double, double& oL, double& oR)
 {
     if constexpr (MODE != MODE_JS) {
         oL = oR = __builtin_nan("");
@@ -227,20 +236,21 @@ __device__ __forceinline__ void weno3(const C* v, const Cf<3, C>& f, double eps,
         const C pL[2] = {v[1] + f.pl[0] * d0, v[1] + f.pl[1] * dm1};
         const C pR[2] = {v[1] + f.pr[0] * d0, v[1] + f.pr[1] * dm1};
         const double D0 = d0, Dm1 = dm1;
-        const double b0 = f.b[0] * D0 * D0 + eps;
-        const double b1 = f.b[1] * Dm1 * Dm1 + eps;
+        const double b0 = B(0, 0) * D0 * D0 + eps;
+        const double b1 = B(1, 0) * Dm1 * Dm1 + eps;
         const double q[2] = {b0 * b0, b1 * b1};
         double P[2];
         excl_products<2>(q, P);
-        oL = js_combine<2, C>(q, P, f.dl, pL);
-        oR = js_combine<2, C>(q, P, f.dr, pR);
+        const double dl[2] = {B.dl(0), B.dl(1)}, dr[2] = {B.dr(0), B.dr(1)};
+        oL = js_combine<2, C>(q, P, dl, pL);
+        oR = js_combine<2, C>(q, P, dr, pR);
     }
 }
 
 // ---- WENO5: v[0..4] = j-2..j+2 (weno_standalone.f90:477-669) -------------
 template <int MODE, typename C>
-__device__ __forceinline__ void weno5(const C* v, const Cf<5, C>& f, double eps, double ct,
-                                      double& oL, double& oR)
+__device__ __forceinline__ void weno5(const C* v, const Cf<5, C>& f, const BetaRef<3>& B, double eps,
+                                      double ct, double& oL, double& oR)
 {
     if constexpr (MODE == MODE_NONE) {
         oL = oR = __builtin_nan("");
@@ -255,15 +265,16 @@ __device__ __forceinline__ void weno5(const C* v, const Cf<5, C>& f, double eps,
                          vj + f.pr[1][0] * d0 + f.pr[1][1] * dm1,
                          vj + f.pr[2][0] * dm1 + f.pr[2][1] * dm2};
         const double Dp1 = dp1, D0 = d0, Dm1 = dm1, Dm2 = dm2;
-        const double b0 = f.b[0][0] * Dp1 * Dp1 + f.b[0][1] * Dp1 * D0 + f.b[0][2] * D0 * D0 + eps;
-        const double b1 = f.b[1][0] * D0 * D0 + f.b[1][1] * D0 * Dm1 + f.b[1][2] * Dm1 * Dm1 + eps;
-        const double b2 = f.b[2][0] * Dm1 * Dm1 + f.b[2][1] * Dm1 * Dm2 + f.b[2][2] * Dm2 * Dm2 + eps;
+        const double b0 = B(0, 0) * Dp1 * Dp1 + B(0, 1) * Dp1 * D0 + B(0, 2) * D0 * D0 + eps;
+        const double b1 = B(1, 0) * D0 * D0 + B(1, 1) * D0 * Dm1 + B(1, 2) * Dm1 * Dm1 + eps;
+        const double b2 = B(2, 0) * Dm1 * Dm1 + B(2, 1) * Dm1 * Dm2 + B(2, 2) * Dm2 * Dm2 + eps;
         if constexpr (MODE == MODE_JS) {
             const double q[3] = {b0 * b0, b1 * b1, b2 * b2};
             double P[3];
             excl_products<3>(q, P);
-            oL = js_combine<3, C>(q, P, f.dl, pL);
-            oR = js_combine<3, C>(q, P, f.dr, pR);
+            const double dl[3] = {B.dl(0), B.dl(1), B.dl(2)}, dr[3] = {B.dr(0), B.dr(1), B.dr(2)};
+            oL = js_combine<3, C>(q, P, dl, pL);
+            oR = js_combine<3, C>(q, P, dr, pR);
         } else {
             // :578-606, kept with IEEE divisions and pow
             const double tau = fabs(b2 - b0);
@@ -274,8 +285,8 @@ __device__ __forceinline__ void weno5(const C* v, const Cf<5, C>& f, double eps,
             const double del0 = (a0 / s < ct) ? 0.0 : 1.0;
             const double del1 = (a1 / s < ct) ? 0.0 : 1.0;
             const double del2 = (a2 / s < ct) ? 0.0 : 1.0;
-            const double aL[3] = {del0 * f.dl[0], del1 * f.dl[1], del2 * f.dl[2]};
-            const double aR[3] = {del0 * f.dr[0], del1 * f.dr[1], del2 * f.dr[2]};
+            const double aL[3] = {del0 * B.dl(0), del1 * B.dl(1), del2 * B.dl(2)};
+            const double aR[3] = {del0 * B.dr(0), del1 * B.dr(1), del2 * B.dr(2)};
             oL = masked_combine<3, C>(aL, pL, 0.0);
             oR = masked_combine<3, C>(aR, pR, 0.0);
         }
@@ -286,8 +297,8 @@ __device__ __forceinline__ void weno5(const C* v, const Cf<5, C>& f, double eps,
 // poly(q)/beta(q) use coefficient row 3-q, alpha(q) uses d(q): the
 // reference's pairing is kept as is.
 template <int MODE, typename C>
-__device__ __forceinline__ void weno7(const C* v, const Cf<7, C>& f, double eps, double ct,
-                                      double& oL, double& oR)
+__device__ __forceinline__ void weno7(const C* v, const Cf<7, C>& f, const BetaRef<6>& B, double eps,
+                                      double ct, double& oL, double& oR)
 {
     if constexpr (MODE == MODE_NONE) {
         oL = oR = __builtin_nan("");
@@ -307,31 +318,33 @@ __device__ __forceinline__ void weno7(const C* v, const Cf<7, C>& f, double eps,
         const double D2 = d2, D1 = d1, D0 = d0, Dm1 = dm1, Dm2 = dm2, Dm3 = dm3;
         double b[4];
         {
-            const double(&c)[6] = f.b[0];
-            b[3] = c[0] * D2 * D2 + c[1] * D2 * D1 + c[2] * D2 * D0 + c[3] * D1 * D1 + c[4] * D1 * D0 +
-                   c[5] * D0 * D0 + eps;
+            auto c = [&](int t) { return B(0, t); };
+            b[3] = c(0) * D2 * D2 + c(1) * D2 * D1 + c(2) * D2 * D0 + c(3) * D1 * D1 + c(4) * D1 * D0 +
+                   c(5) * D0 * D0 + eps;
         }
         {
-            const double(&c)[6] = f.b[1];
-            b[2] = c[0] * D1 * D1 + c[1] * D1 * D0 + c[2] * D1 * Dm1 + c[3] * D0 * D0 + c[4] * D0 * Dm1 +
-                   c[5] * Dm1 * Dm1 + eps;
+            auto c = [&](int t) { return B(1, t); };
+            b[2] = c(0) * D1 * D1 + c(1) * D1 * D0 + c(2) * D1 * Dm1 + c(3) * D0 * D0 + c(4) * D0 * Dm1 +
+                   c(5) * Dm1 * Dm1 + eps;
         }
         {
-            const double(&c)[6] = f.b[2];
-            b[1] = c[0] * D0 * D0 + c[1] * D0 * Dm1 + c[2] * D0 * Dm2 + c[3] * Dm1 * Dm1 +
-                   c[4] * Dm1 * Dm2 + c[5] * Dm2 * Dm2 + eps;
+            auto c = [&](int t) { return B(2, t); };
+            b[1] = c(0) * D0 * D0 + c(1) * D0 * Dm1 + c(2) * D0 * Dm2 + c(3) * Dm1 * Dm1 +
+                   c(4) * Dm1 * Dm2 + c(5) * Dm2 * Dm2 + eps;
         }
         {
-            const double(&c)[6] = f.b[3];
-            b[0] = c[0] * Dm1 * Dm1 + c[1] * Dm1 * Dm2 + c[2] * Dm1 * Dm3 + c[3] * Dm2 * Dm2 +
-                   c[4] * Dm2 * Dm3 + c[5] * Dm3 * Dm3 + eps;
+            auto c = [&](int t) { return B(3, t); };
+            b[0] = c(0) * Dm1 * Dm1 + c(1) * Dm1 * Dm2 + c(2) * Dm1 * Dm3 + c(3) * Dm2 * Dm2 +
+                   c(4) * Dm2 * Dm3 + c(5) * Dm3 * Dm3 + eps;
         }
         if constexpr (MODE == MODE_JS) {
             const double q[4] = {b[0] * b[0], b[1] * b[1], b[2] * b[2], b[3] * b[3]};
             double P[4];
             excl_products<4>(q, P);
-            oL = js_combine<4, C>(q, P, f.dl, pL);
-            oR = js_combine<4, C>(q, P, f.dr, pR);
+            const double dl[4] = {B.dl(0), B.dl(1), B.dl(2), B.dl(3)};
+            const double dr[4] = {B.dr(0), B.dr(1), B.dr(2), B.dr(3)};
+            oL = js_combine<4, C>(q, P, dl, pL);
+            oR = js_combine<4, C>(q, P, dr, pR);
         } else {
             // :1490-1503: beta(4) keeps weno_eps, alpha(4) is never cut
             const double tau = fabs(eps - b[3]);
@@ -349,8 +362,8 @@ __device__ __forceinline__ void weno7(const C* v, const Cf<7, C>& f, double eps,
             double del[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) del[q] = (a[q] / s < ct) ? 0.0 : 1.0;
-            const double aL[4] = {del[0] * f.dl[0], del[1] * f.dl[1], del[2] * f.dl[2], del[3] * f.dl[3]};
-            const double aR[4] = {del[0] * f.dr[0], del[1] * f.dr[1], del[2] * f.dr[2], del[3] * f.dr[3]};
+            const double aL[4] = {del[0] * B.dl(0), del[1] * B.dl(1), del[2] * B.dl(2), del[3] * B.dl(3)};
+            const double aR[4] = {del[0] * B.dr(0), del[1] * B.dr(1), del[2] * B.dr(2), del[3] * B.dr(3)};
             oL = masked_combine<4, C>(aL, pL, a[4]);
             oR = masked_combine<4, C>(aR, pR, a[4]);
         }
@@ -405,19 +418,36 @@ __device__ __forceinline__ void mp_limit(const C* v, C& vL, C& vR)
     }
 }
 
-constexpr int kMaxNP = 32;
-constexpr int kBlock = 256;
+constexpr int kMaxThreads = 1024;
+constexpr int kMaxNP = 64;
+constexpr int kMaxSlots = 6;
 
 template <int ORDER>
 struct OrderTraits {
     static constexpr int R = ORDER / 2;
 };
 
+// WENO7 keeps 56 coefficients per lane in registers: give its waves the
+// 256-VGPR budget of two waves per SIMD (512-thread blocks); everything else
+// runs four waves per SIMD (1024-thread blocks, 128 VGPRs).
+template <int OP, int ORDER>
+struct BlockTraits {
+    static constexpr int kThreads = (OP == OP_WENO && ORDER == 7) ? 512 : kMaxThreads;
+};
+
 // ---------------------------------------------------------------------------
-// the sweep kernel
+// the sweep kernel: persistent, double-buffered
+//
+//   block = G lane groups; group g owns pencils [g*npg, (g+1)*npg) of a tile
+//   and lane jj of the group owns output j = jlo + jj for the whole launch, so
+//   the per-j coefficients are loaded once per thread.
+//   Each block walks a contiguous-per-XCD range of tiles.  While tile k is
+//   computed from LDS buffer k&1, tile k+1 is already in flight from HBM into
+//   registers and is written to the other buffer afterwards: one barrier per
+//   tile.  Pencil address tables are built two tiles ahead (3 slots).
 // ---------------------------------------------------------------------------
 template <int OP, int ORDER, int MODE, bool MP, typename TIN, typename TOUT>
-__global__ void __launch_bounds__(kBlock) sweep_kernel(const TileArgs a, const int np, const int nt)
+__global__ void __launch_bounds__((BlockTraits<OP, ORDER>::kThreads)) sweep_kernel(const TileArgs a, const int nt)
 {
     using C = typename CT<TIN>::type;
     constexpr int RW = (OP == OP_WENO) ? OrderTraits<ORDER>::R : 0;
@@ -425,31 +455,35 @@ __global__ void __launch_bounds__(kBlock) sweep_kernel(const TileArgs a, const i
     constexpr int W = 2 * R + 1;
 
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    C* lds = reinterpret_cast<C*>(smem_raw);
-    __shared__ long long s_in[kMaxNP];
-    __shared__ long long s_out[kMaxNP];
+    C* const lds0 = reinterpret_cast<C*>(smem_raw);
+    __shared__ long long s_in[3][kMaxNP];
+    __shared__ long long s_out[3][kMaxNP];
 
     const int tid = threadIdx.x;
-    const int t = xcd_remap(blockIdx.x, gridDim.x);
-    const int per_var = a.tiles_per_var * a.chunks;
-    const int iv = t / per_var;
-    const int rem = t - iv * per_var;
-    const int grp = rem / a.chunks;
-    const int ch = rem - grp * a.chunks;
-    const int npen = a.na * a.nb;
-    const int P0 = grp * np;
-    const int nq = min(np, npen - P0);
+    const int T = blockDim.x;
+    const int np = a.np;
+    const int ldj = a.ldj;
+    C* const lds1 = lds0 + np * ldj;
+    constexpr int OB = (ORDER == 3 || ORDER == 5 || ORDER == 7) ? ORDER : 5;
+    constexpr int NS = BetaTraits<OB>::NS;
+    constexpr int NB = (OP == OP_WENO) ? BetaTraits<OB>::N : 0;
+    constexpr int NR = BetaTraits<OB>::NR;
+    double* const beta_lds =
+        reinterpret_cast<double*>(smem_raw + ((2 * static_cast<size_t>(np) * ldj * sizeof(C) + 15) & ~size_t(15)));
 
-    if (tid < nq) {
-        const int P = P0 + tid;
-        const int pa = P % a.na, pb = P / a.na;
-        s_in[tid] = a.src_off + pa * a.s_a + pb * a.s_b + iv * a.s_i;
-        const int k = a.a_is_k ? pa : pb;
-        const int l = a.a_is_k ? pb : pa;
-        s_out[tid] = k * a.o_k + l * a.o_l + iv * a.o_i;
-    }
+    // ---- which chunk and which tiles ----
+    const int bpc = a.bpc;
+    const int ch = blockIdx.x / bpc;
+    const int bin = blockIdx.x - ch * bpc;
+    const int ntile = a.tiles_per_var * a.nv;
+    const int xcd = bin & 7, rank = bin >> 3, nbx = bpc >> 3;
+    const int per = ntile >> 3, rem = ntile & 7;
+    const int t_first = xcd * per + min(xcd, rem) + rank;
+    const int t_end = (xcd + 1) * per + min(xcd + 1, rem);
+    const int t_stride = nbx;
+    if (t_first >= t_end) return;
 
-    // output range of this chunk and the input range it needs
+    // ---- output range of the chunk and the input range it needs ----
     const int jo0 = ch * a.chunk;
     const int jo1 = min(a.out_len, jo0 + a.chunk);
     const int c0 = max(jo0, a.j_beg), c1 = min(jo1 - 1, a.j_end);
@@ -472,96 +506,214 @@ __global__ void __launch_bounds__(kBlock) sweep_kernel(const TileArgs a, const i
         jhi = c1;
     }
     const int nJ = hi - lo + 1;
-    const int ldj = a.ldj;
-    __syncthreads();
 
-    // ---- load phase ----
-    const TIN* src = static_cast<const TIN*>(a.src);
-    if (a.s_j == 1) {
-        for (int q = 0; q < nq; ++q) {
-            const TIN* row = src + s_in[q] + lo;
-            C* dst = lds + q * ldj;
-            for (int J = tid; J < nJ; J += kBlock) dst[J] = static_cast<C>(ld_in(row + J));
-        }
-    } else {
-        const int q = tid & (np - 1);
-        const int step = kBlock / np;
-        if (q < nq) {
-            const TIN* col = src + s_in[q] + static_cast<long long>(lo) * a.s_j;
-            C* dst = lds + q * ldj;
-            for (int J = tid / np; J < nJ; J += step)
-                dst[J] = static_cast<C>(ld_in(col + static_cast<long long>(J) * a.s_j));
+    // ---- fixed per-thread roles ----
+    const int gq = tid / a.jn;
+    const int j = jlo + (tid - gq * a.jn);
+    const bool active = gq < a.groups && j <= jhi;
+    const bool core = active && has_core && j >= c0 && j <= c1;
+    const bool do_mp = MP && core && j >= a.mp_beg && j <= a.mp_end;
+    const int col = j + a.J0f - lo;
+    const int q0 = gq * a.npg;
+
+    // load slots of this thread (same for every tile): (q << 16) | J, or -1
+    int slot[kMaxSlots];
+    const int nel = np * nJ;
+#pragma unroll
+    for (int s = 0; s < kMaxSlots; ++s) {
+        const int e = tid + s * T;
+        slot[s] = -1;
+        if (s < a.slots && e < nel) {
+            int q, J;
+            if (a.s_j == 1) {
+                q = e / nJ;
+                J = e - q * nJ;
+            } else {
+                J = e / np;
+                q = e - J * np;
+            }
+            slot[s] = (q << 16) | J;
         }
     }
-    __syncthreads();
 
-    // ---- compute / store phase ----
-    TOUT* outL = static_cast<TOUT*>(a.outL);
-    TOUT* outR = static_cast<TOUT*>(a.outR);
-    const bool ntb = nt != 0;
-    for (int j = jlo + tid; j <= jhi; j += kBlock) {
-        const int col = j + a.J0f - lo;
-        const bool core = has_core && j >= c0 && j <= c1;
-        if (!core) {
-            for (int q = 0; q < nq; ++q) {
-                const C val = lds[q * ldj + col];
-                st_out(outL + s_out[q] + j, val, ntb);
-                if constexpr (OP != OP_RESHAPE) st_out(outR + s_out[q] + j, val, ntb);
-            }
-            continue;
-        }
-        if constexpr (OP == OP_WENO) {
+    Cf<OB, C> f;
+    const int jj = tid - gq * a.jn;
+    if constexpr (OP == OP_WENO) {
+        if (core) {
             int jc = j - 3;
             jc = jc < 0 ? 0 : (jc > a.c.nc - 1 ? a.c.nc - 1 : jc);
-            Cf<ORDER, C> f;
             load_cf(a.c, jc, f);
-            for (int q = 0; q < nq; ++q) {
-                C v[W];
+            if (gq == 0) {
 #pragma unroll
-                for (int o = 0; o < W; ++o) v[o] = lds[q * ldj + col - R + o];
-                double oL, oR;
-                const C* vc = v + (R - RW);
-                if constexpr (ORDER == 3)
-                    weno3<MODE, C>(vc, f, a.eps, a.teno_ct, oL, oR);
-                else if constexpr (ORDER == 5)
-                    weno5<MODE, C>(vc, f, a.eps, a.teno_ct, oL, oR);
-                else
-                    weno7<MODE, C>(vc, f, a.eps, a.teno_ct, oL, oR);
-                if constexpr (MP) {
-                    C xl = static_cast<C>(oL), xr = static_cast<C>(oR);
-                    mp_limit<C>(v + (R - 2), xl, xr);
-                    st_out(outL + s_out[q] + j, xl, ntb);
-                    st_out(outR + s_out[q] + j, xr, ntb);
-                } else {
-                    st_out(outL + s_out[q] + j, oL, ntb);
-                    st_out(outR + s_out[q] + j, oR, ntb);
+                for (int t = 0; t < NB; ++t) {
+                    const int r = t / NS, s = t - (t / NS) * NS;
+                    beta_lds[t * a.jn + jj] = a.c.beta[jc + a.c.nc * (r + 5 * s)];
+                }
+#pragma unroll
+                for (int r = 0; r < NR; ++r) {
+                    beta_lds[(NB + r) * a.jn + jj] = a.c.dL[r + 5 * jc];
+                    beta_lds[(NB + NR + r) * a.jn + jj] = a.c.dR[r + 5 * jc];
                 }
             }
-        } else if constexpr (OP == OP_MP) {
-            for (int q = 0; q < nq; ++q) {
-                C v[W];
-#pragma unroll
-                for (int o = 0; o < W; ++o) v[o] = lds[q * ldj + col - R + o];
-                C xl = static_cast<C>(outL[s_out[q] + j]);
-                C xr = static_cast<C>(outR[s_out[q] + j]);
-                mp_limit<C>(v, xl, xr);
-                st_out(outL + s_out[q] + j, xl, false);
-                st_out(outR + s_out[q] + j, xr, false);
+        }
+    }
+
+    const TIN* const src = static_cast<const TIN*>(a.src);
+    TOUT* const outL = static_cast<TOUT*>(a.outL);
+    TOUT* const outR = static_cast<TOUT*>(a.outR);
+    const int npen = a.na * a.nb;
+    const bool ntb = nt != 0;
+    const bool wl = (a.sides & 1) != 0, wr = (a.sides & 2) != 0;
+
+    auto tables = [&](int t, int slot) {
+        if (tid < np) {
+            const int iv = t / a.tiles_per_var;
+            const int P = (t - iv * a.tiles_per_var) * np + tid;
+            if (P < npen) {
+                const int pa = P % a.na, pb = P / a.na;
+                s_in[slot][tid] = a.src_off + pa * a.s_a + pb * a.s_b + iv * a.s_i + lo * a.s_j;
+                const int k = a.a_is_k ? pa : pb;
+                const int l = a.a_is_k ? pb : pa;
+                s_out[slot][tid] = k * a.o_k + l * a.o_l + iv * a.o_i;
+            } else {
+                s_in[slot][tid] = -1;
+                s_out[slot][tid] = -1;
             }
         }
+    };
+    C regs[kMaxSlots];
+    auto issue = [&](int slot_id) {
+#pragma unroll
+        for (int s = 0; s < kMaxSlots; ++s) {
+            regs[s] = C(0);
+            // opaque copy: keeps per-slot 64-bit addresses from being hoisted
+            // out of the tile loop into registers
+            int sl = slot[s];
+            asm volatile("" : "+v"(sl));
+            if (sl >= 0) {
+                const long long b = s_in[slot_id][sl >> 16];
+                if (b >= 0) regs[s] = static_cast<C>(ld_in(src + b + static_cast<long long>(sl & 0xffff) * a.s_j));
+            }
+        }
+    };
+    auto commit = [&](C* buf) {
+#pragma unroll
+        for (int s = 0; s < kMaxSlots; ++s)
+            if (slot[s] >= 0) buf[(slot[s] >> 16) * ldj + (slot[s] & 0xffff)] = regs[s];
+    };
+
+    // ---- prologue ----
+    tables(t_first, 0);
+    if (t_first + t_stride < t_end) tables(t_first + t_stride, 1);
+    __syncthreads();
+    issue(0);
+    commit(lds0);
+    __syncthreads();
+
+    int k = 0;
+    for (int t = t_first; t < t_end; t += t_stride, ++k) {
+        C* const cur = (k & 1) ? lds1 : lds0;
+        C* const nxt = (k & 1) ? lds0 : lds1;
+        const int tslot = k % 3;
+        const bool more = t + t_stride < t_end;
+        if (more) issue((k + 1) % 3);
+        if (t + 2 * t_stride < t_end) tables(t + 2 * t_stride, (k + 2) % 3);
+
+        if (active) {
+#pragma unroll 1
+            for (int qq = 0; qq < a.npg; ++qq) {
+                const int q = q0 + qq;
+                const long long ob = s_out[tslot][q];
+                if (ob < 0) continue;
+                const C* row = cur + q * ldj + col;
+                if (!core) {
+                    const C val = row[0];
+                    if (wl) st_out(outL + ob + j, val, ntb);
+                    if constexpr (OP != OP_RESHAPE)
+                        if (wr) st_out(outR + ob + j, val, ntb);
+                    continue;
+                }
+                if constexpr (OP == OP_WENO) {
+                    // re-read the LDS coefficient table per pencil: an opaque
+                    // pointer keeps the loads from being hoisted into registers
+                    const double* bp = beta_lds + jj;
+                    asm volatile("" : "+v"(bp));
+                    const BetaRef<NS> B{bp, a.jn, NB, NR};
+                    C v[W];
+#pragma unroll
+                    for (int o = 0; o < W; ++o) v[o] = row[o - R];
+                    double oL, oR;
+                    const C* vc = v + (R - RW);
+                    if constexpr (ORDER == 3)
+                        weno3<MODE, C>(vc, f, B, a.eps, a.teno_ct, oL, oR);
+                    else if constexpr (ORDER == 5)
+                        weno5<MODE, C>(vc, f, B, a.eps, a.teno_ct, oL, oR);
+                    else
+                        weno7<MODE, C>(vc, f, B, a.eps, a.teno_ct, oL, oR);
+                    if constexpr (MP) {
+                        C xl = static_cast<C>(oL), xr = static_cast<C>(oR);
+                        if (do_mp) mp_limit<C>(v + (R - 2), xl, xr);
+                        if (wl) st_out(outL + ob + j, xl, ntb);
+                        if (wr) st_out(outR + ob + j, xr, ntb);
+                    } else {
+                        if (wl) st_out(outL + ob + j, oL, ntb);
+                        if (wr) st_out(outR + ob + j, oR, ntb);
+                    }
+                } else if constexpr (OP == OP_MP) {
+                    C v[W];
+#pragma unroll
+                    for (int o = 0; o < W; ++o) v[o] = row[o - R];
+                    C xl = static_cast<C>(outL[ob + j]);
+                    C xr = static_cast<C>(outR[ob + j]);
+                    mp_limit<C>(v, xl, xr);
+                    st_out(outL + ob + j, xl, false);
+                    st_out(outR + ob + j, xr, false);
+                }
+            }
+        }
+        if (more) commit(nxt);
+        __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------------------
 // host dispatch
 // ---------------------------------------------------------------------------
+static int cu_count()
+{
+    static int n = 0;
+    if (n == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+            n = 256;
+    }
+    return n;
+}
+
 template <int OP, int ORDER, int MODE, bool MP, typename TIN, typename TOUT>
-static hipError_t go(const TileArgs& a, const Launch& l, hipStream_t st)
+static hipError_t go(TileArgs a, const Launch& l, hipStream_t st)
 {
     using C = typename CT<TIN>::type;
-    const size_t lds = static_cast<size_t>(l.np) * a.ldj * sizeof(C);
-    hipLaunchKernelGGL((sweep_kernel<OP, ORDER, MODE, MP, TIN, TOUT>), dim3(a.ntiles), dim3(kBlock), lds,
-                       st, a, l.np, l.nontemporal);
+    auto kern = sweep_kernel<OP, ORDER, MODE, MP, TIN, TOUT>;
+    constexpr int NB = (OP == OP_WENO) ? BetaTraits<ORDER>::ENTRIES : 0;
+    const size_t lds = ((2 * static_cast<size_t>(a.np) * a.ldj * sizeof(C) + 15) & ~size_t(15)) +
+                       static_cast<size_t>(NB) * a.jn * sizeof(double);
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+        if (e != hipSuccess) return e;
+    }
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, a.threads, lds) != hipSuccess || occ < 1) occ = 1;
+    const int ntile = a.tiles_per_var * a.nv;
+    int bpc = cu_count() * occ;
+    if (l.blocks_per_cu > 0) bpc = cu_count() * l.blocks_per_cu;
+    const int need = (ntile + 1) / 2;    // at least two tiles per block
+    if (bpc > need) bpc = need;
+    bpc = (bpc + 7) & ~7;
+    a.bpc = bpc;
+    hipLaunchKernelGGL(kern, dim3(bpc * a.chunks), dim3(a.threads), lds, st, a, l.nontemporal);
     return hipGetLastError();
 }
 
@@ -584,9 +736,14 @@ static hipError_t go_weno(const TileArgs& a, const Launch& l, hipStream_t st)
     return hipErrorInvalidValue;
 }
 
+int max_threads(int op, int order);
+
 hipError_t launch_tile(const TileArgs& a, const Launch& l, hipStream_t st)
 {
-    if (a.ntiles <= 0) return hipSuccess;
+    if (a.tiles_per_var <= 0 || a.nv <= 0) return hipSuccess;
+    if (a.np > kMaxNP || a.slots > kMaxSlots || a.threads > max_threads(l.op, l.order) || a.threads < 64 ||
+        a.ldj > 0xffff)
+        return hipErrorInvalidConfiguration;
     switch (l.op) {
     case OP_RESHAPE:
         if (l.dtype == DT_F64) return go<OP_RESHAPE, 5, MODE_JS, false, double, double>(a, l, st);
@@ -607,5 +764,9 @@ hipError_t launch_tile(const TileArgs& a, const Launch& l, hipStream_t st)
     }
     return hipErrorInvalidValue;
 }
+
+int kernel_max_slots() { return kMaxSlots; }
+int max_threads(int op, int order) { return (op == OP_WENO && order == 7) ? 512 : kMaxThreads; }
+int kernel_max_np() { return kMaxNP; }
 
 }  // namespace wm
