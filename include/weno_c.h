@@ -141,6 +141,26 @@ int weno_reconstruct(const weno_grid* g, int order, int dir, int src_kind, int d
 int weno_mp_limit(const weno_grid* g, int dir, int src_kind, int dtype, const void* src,
                   void* vL, void* vR);
 
+/* Adjoint of weno_reconstruct for order 5/7 with JS weights (wenojs set),
+ * FP64 (no reference counterpart: the reference has no derivative).  For
+ * cotangents gL, gR on the core outputs of one sweep (face layout), adds
+ *   (dvL/dsrc)^T gL + (dvR/dsrc)^T gR
+ * to gv, which has src's layout (v_vf or v_rs_ws_dir) and must be zeroed by
+ * the caller before the first sweep; beta and the weights are recomputed,
+ * nothing is stored by the forward pass.  Sweeps of one field accumulate. */
+int weno_reconstruct_vjp(const weno_grid* g, int order, int dir, int src_kind, const void* src,
+                         const weno_coeffs* c, const void* gL, const void* gR, void* gv);
+
+/* Plain device-to-device streaming copy (16-byte multiples): the bandwidth
+ * yardstick for the sweeps' HBM traffic counters.  No reference counterpart. */
+int weno_stream_copy(void* dst, const void* src, size_t bytes);
+
+/* Tile shape of the sweep kernel for (order, dtype, dir): pencils per tile,
+ * outputs per tile along the sweep, threads per workgroup (0 = automatic
+ * for each) and prefetch of the next tile during compute (-1 = automatic,
+ * 0 off, 1 on).  Used by the autotuner; no reference counterpart. */
+int weno_set_tile(int order, int dtype, int dir, int pencils, int chunk, int threads, int prefetch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -85,7 +85,8 @@ def test_shapes():
 
 
 def test_set_tile_validation():
-    assert wm.lib.weno_set_tile(5, 0, 0, 3, 0) == -1      # not a power of two
-    assert wm.lib.weno_set_tile(6, 0, 0, 16, 0) == -1
-    assert wm.lib.weno_set_tile(5, 0, 0, 16, 0) == 0
-    assert wm.lib.weno_set_tile(5, 0, 0, 0, 0) == 0
+    assert wm.lib.weno_set_tile(5, 0, 0, 65, 0, 0, -1) == -1
+    assert wm.lib.weno_set_tile(6, 0, 0, 16, 0, 0, -1) == -1
+    assert wm.lib.weno_set_tile(5, 0, 0, 16, 0, 100, -1) == -1     # not a multiple of 64
+    assert wm.lib.weno_set_tile(5, 0, 0, 16, 0, 448, 0) == 0
+    assert wm.lib.weno_set_tile(5, 0, 0, 0, 0, 0, -1) == 0

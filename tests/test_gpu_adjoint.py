@@ -1,0 +1,142 @@
+"""Adjoint of WENO5/7-JS on the GPU (weno_reconstruct_vjp), FP64.
+
+Checked two independent ways:
+  * against the oracle's FP64 CPU gradient (oracle_weno_vjp, itself pinned by
+    a complex-step derivative in tests/test_oracle.py), normwise
+    (max|a-b| <= tol * max|b|): each gradient entry sums terms of both signs
+    that scale like 1/beta, so entries that nearly cancel carry the rounding
+    of the larger terms and an element-wise relative bound is meaningless.
+    tol = 1e-12 on smooth data; 1e-10 on the noisy seeded field, whose 1e-3
+    noise makes beta ~ 1e-6 and the cancelling terms ~1e4 x the result;
+  * against central finite differences (h = 1e-6) of the GPU forward on
+    cells of a 16^3 sub-block: 1e-6 relative (FD truncation and roundoff).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle
+import wenostandalone_amd as wm
+from wenostandalone_amd import fields
+from wenostandalone_amd.device import DeviceArray
+pytestmark = pytest.mark.gpu
+
+
+def norm_err(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _init():
+    wm.check(wm.lib.weno_init(0))
+
+
+def field_from_reshaped_grad(gv_rs, d, nx, ny, nz, v):
+    """Scatter of the reshape (a permutation of a sub-box of v_vf)."""
+    out = np.zeros((nx + 6, ny + 6, nz + 6, v), order="F")
+    if d == 0:
+        out[:, 3:3 + ny, 3:3 + nz, :] = gv_rs
+    elif d == 1:
+        out[3:3 + nx, :, 3:3 + nz, :] = gv_rs.transpose(1, 0, 2, 3)
+    else:
+        out[3:3 + nx, 3:3 + ny, :, :] = gv_rs.transpose(2, 1, 0, 3)
+    return out
+
+
+def cotangents(shape, seed):
+    rng = np.random.default_rng(seed)
+    gL = np.asfortranarray(rng.standard_normal(shape))
+    gR = np.asfortranarray(rng.standard_normal(shape))
+    gL[:3] = gL[-3:] = 0.0
+    gR[:3] = gR[-3:] = 0.0
+    return gL, gR
+
+
+def smooth_field(m, n, p, v, b):
+    nx, ny, nz = fields.extents(m, n, p, b)
+    x = np.linspace(0, 1, nx + 6)[:, None, None, None]
+    y = np.linspace(0, 1, ny + 6)[None, :, None, None]
+    z = np.linspace(0, 1, nz + 6)[None, None, :, None]
+    k = np.arange(1, v + 1)[None, None, None, :]
+    return np.asfortranarray(1.5 + np.sin(2.0 * x * k) * np.cos(1.3 * y) + 0.3 * np.sin(2.2 * z + k))
+
+
+@pytest.mark.parametrize("smooth", [True, False])
+@pytest.mark.parametrize("order", [5, 7])
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_vjp_vs_oracle(order, d, smooth):
+    m, n, p, v, b = 9, 12, 7, 2, 6
+    g = wm.make_grid(m, n, p, v=v, buff=b)
+    og = oracle.grid(m, n, p, v=v, buff=b)
+    f = smooth_field(m, n, p, v, b) if smooth else fields.seeded_field(m, n, p, v, b, seed=60 + d)
+    tol = 1e-12 if smooth else 1e-10
+    nx, ny, nz = fields.extents(m, n, p, b)
+    ns, nk, nl = fields.dir_extents(m, n, p, b, d)
+    cs = fields.random_coefficients(ns - 6, seed=order + d)
+    devs = [DeviceArray.from_host(c) for c in cs]
+    coef = wm.Coeffs(*[x.ptr for x in devs])
+    gL, gR = cotangents((ns, nk, nl, v), seed=d)
+    v_rs = oracle.reshape(og, d, f)
+    ref_rs = oracle.vjp(og, order, d, v_rs, cs, gL, gR)
+    dgL, dgR = DeviceArray.from_host(gL), DeviceArray.from_host(gR)
+    # reshaped input layout
+    rs = DeviceArray.from_host(v_rs)
+    gv = DeviceArray(v_rs.shape)
+    gv.fill(0.0)
+    wm.check(wm.lib.weno_reconstruct_vjp(C.byref(g), order, d, wm.SRC_RESHAPED, rs.vp, C.byref(coef),
+                                         dgL.vp, dgR.vp, gv.vp))
+    assert norm_err(gv.download(), ref_rs) <= tol
+    # padded-field layout, accumulated onto a non-zero gradient
+    vf = DeviceArray.from_host(f)
+    base = np.full(f.shape, 0.5, order="F")
+    gvf = DeviceArray.from_host(base)
+    wm.check(wm.lib.weno_reconstruct_vjp(C.byref(g), order, d, wm.SRC_FIELD, vf.vp, C.byref(coef),
+                                         dgL.vp, dgR.vp, gvf.vp))
+    exp = base + field_from_reshaped_grad(ref_rs, d, nx, ny, nz, v)
+    assert norm_err(gvf.download() - base, exp - base) <= tol
+
+
+@pytest.mark.parametrize("order", [5, 7])
+def test_vjp_central_differences_16cube(order):
+    """A 16^3 block (m=n=p=3, buff 6 -> 16^3 `is` box), x sweep: VJP entries
+    against central differences of the GPU forward on 48 sampled cells."""
+    m, v, b = 3, 1, 6
+    g = wm.make_grid(m, v=v, buff=b)
+    f = fields.seeded_field(m, m, m, v, b, seed=77)
+    ns, nk, nl = fields.dir_extents(m, m, m, b, 0)
+    cs = fields.harness_coefficients(ns - 6, 0.03)
+    devs = [DeviceArray.from_host(c) for c in cs]
+    coef = wm.Coeffs(*[x.ptr for x in devs])
+    gL, gR = cotangents((ns, nk, nl, v), seed=5)
+    vf = DeviceArray.from_host(f)
+    gv = DeviceArray(f.shape)
+    gv.fill(0.0)
+    dgL, dgR = DeviceArray.from_host(gL), DeviceArray.from_host(gR)
+    wm.check(wm.lib.weno_reconstruct_vjp(C.byref(g), order, 0, wm.SRC_FIELD, vf.vp, C.byref(coef),
+                                         dgL.vp, dgR.vp, gv.vp))
+    grad = gv.download()
+    L = DeviceArray((ns, nk, nl, v))
+    R = DeviceArray((ns, nk, nl, v))
+
+    def fwd(field):
+        vf.upload(field)
+        wm.check(wm.lib.weno_reconstruct(C.byref(g), order, 0, wm.SRC_FIELD, wm.F64, vf.vp, C.byref(coef),
+                                         L.vp, R.vp, wm.FUSE_COPY))
+        return L.download(), R.download()
+
+    rng = np.random.default_rng(1)
+    h = 1e-6
+    for _ in range(48):
+        X = int(rng.integers(1, ns + 5))
+        Y = int(rng.integers(3, 3 + nk))
+        Z = int(rng.integers(3, 3 + nl))
+        fp = f.copy(order="F")
+        fm = f.copy(order="F")
+        fp[X, Y, Z, 0] += h
+        fm[X, Y, Z, 0] -= h
+        Lp, Rp = fwd(fp)
+        Lm, Rm = fwd(fm)
+        core = (slice(3, ns - 3), slice(None), slice(None), slice(None))
+        fd = (np.sum(gL[core] * (Lp - Lm)[core]) + np.sum(gR[core] * (Rp - Rm)[core])) / (2 * h)
+        assert abs(grad[X, Y, Z, 0] - fd) <= 1e-6 * max(1.0, abs(fd)), (X, Y, Z, grad[X, Y, Z, 0], fd)

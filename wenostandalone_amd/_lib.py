@@ -86,7 +86,9 @@ PROTOTYPES = {
     "weno_order1_copy": (_i, [_G, _i, _i, _vp, _vp, _vp]),
     "weno_reconstruct": (_i, [_G, _i, _i, _i, _i, _vp, _CF, _vp, _vp, _i]),
     "weno_mp_limit": (_i, [_G, _i, _i, _i, _vp, _vp, _vp]),
-    "weno_set_tile": (_i, [_i, _i, _i, _i, _i]),
+    "weno_set_tile": (_i, [_i, _i, _i, _i, _i, _i, _i]),
+    "weno_reconstruct_vjp": (_i, [_G, _i, _i, _i, _vp, _CF, _vp, _vp, _vp]),
+    "weno_stream_copy": (_i, [_vp, _vp, _sz]),
 }
 
 for _name, (_res, _args) in PROTOTYPES.items():

@@ -1,0 +1,81 @@
+// extern "C" entry point of the WENO5/7-JS adjoint (weno_reconstruct_vjp).
+#include "weno_capi_internal.h"
+
+#include <cstring>
+
+namespace wm {
+hipError_t launch_vjp(TileArgs a, int order, double* grad, hipStream_t st);
+int vjp_max_threads();
+int vjp_max_slots();
+int vjp_max_np();
+}  // namespace wm
+
+using namespace wm;
+
+extern "C" int weno_reconstruct_vjp(const weno_grid* g, int order, int dir, int src_kind, const void* src,
+                                    const weno_coeffs* c, const void* gL, const void* gR, void* gv)
+{
+    int rc;
+    if ((rc = check_grid(g)) || (rc = check_dir(dir))) return rc;
+    if (order != 5 && order != 7) return fail(WENO_E_INVALID, "the adjoint covers WENO5 and WENO7 (got order %d)", order);
+    if (!g->wenojs) return fail(WENO_E_INVALID, "the adjoint covers the JS weights only (wenojs must be set)");
+    if (src_kind != WENO_SRC_FIELD && src_kind != WENO_SRC_RESHAPED)
+        return fail(WENO_E_INVALID, "src_kind must be WENO_SRC_FIELD or WENO_SRC_RESHAPED");
+    if ((rc = ensure_init())) return rc;
+    const Dims d = dims_of(g, dir);
+    TileArgs a;
+    std::memset(&a, 0, sizeof a);
+    const void *s, *l, *r;
+    void* o;
+    if ((rc = resolve(src, &s, "src")) || (rc = resolve(gL, &l, "gL")) || (rc = resolve(gR, &r, "gR")) ||
+        (rc = resolve_mut(gv, &o, "gv")))
+        return rc;
+    if ((rc = resolve_coeffs(c, a.c, d.ns - 6))) return rc;
+    input_map(d, dir, src_kind, a);
+    output_map(d, d.ns, a);
+    a.src = s;
+    a.outL = const_cast<void*>(l);
+    a.outR = const_cast<void*>(r);
+    a.J0f = 3;
+    a.j_beg = 3;
+    a.j_end = d.ns - 4;
+    a.chunk = d.ns;
+    a.chunks = 1;
+    a.eps = g->weno_eps;
+    const int R = order / 2, W = order;
+    const int jn = a.j_end - a.j_beg + 1;
+    const int nJ = jn + 2 * R;
+    const int tmax = vjp_max_threads();
+    if (jn > tmax) return fail(WENO_E_INVALID, "adjoint sweep length %d exceeds %d core points", jn, tmax);
+    int G = 1, T = 64 * ((jn + 63) / 64);
+    double best = double(jn) / T;
+    for (int gg = 2; gg <= 16; ++gg) {
+        const int t = 64 * ((gg * jn + 63) / 64);
+        if (t > tmax) break;
+        if (double(gg * jn) / t > best + 1e-9) {
+            best = double(gg * jn) / t;
+            G = gg;
+            T = t;
+        }
+    }
+    const int entries = (order == 5) ? 15 : 32;
+    const int ldj = nJ | 1;
+    const size_t lds_cap = 150 * 1024;
+    int npg = std::max(1, 8 / G);
+    auto lds_of = [&](int np) {
+        return (2 * size_t(np) * ldj + size_t(G) * W * jn + size_t(entries) * jn) * sizeof(double);
+    };
+    while (npg > 1 && (lds_of(G * npg) > lds_cap || (size_t(G) * npg * ldj + T - 1) / T > size_t(vjp_max_slots())))
+        --npg;
+    if (lds_of(G * npg) > lds_cap || (size_t(G) * npg * ldj + T - 1) / T > size_t(vjp_max_slots()))
+        return fail(WENO_E_INVALID, "adjoint tile does not fit LDS for sweep length %d", d.ns);
+    a.groups = G;
+    a.npg = npg;
+    a.np = G * npg;
+    a.jn = jn;
+    a.threads = T;
+    a.ldj = ldj;
+    a.slots = static_cast<int>((size_t(a.np) * ldj + T - 1) / T);
+    a.tiles_per_var = static_cast<int>((static_cast<long long>(a.na) * a.nb + a.np - 1) / a.np);
+    return finish(launch_vjp(a, order, static_cast<double*>(o), stream()), "reconstruct_vjp");
+}

@@ -178,8 +178,14 @@ void output_map(const Dims& d, int row_len, TileArgs& a)
     a.o_i = static_cast<long long>(row_len) * d.nk * d.nl;
 }
 
+// Defaults from tile-shape sweeps on MI355X at 200^3 x 8 (FP64 WENO5): the
+// persistent loop with the coefficients in registers and no prefetch, in
+// 448-thread workgroups (two lane groups of 206 + 2 workgroups per CU).
+constexpr int kDefaultVariant = 2;
+constexpr int kDefaultWeno5Threads = 448;
+
 // Pencils per tile, j-chunking and LDS row stride for one launch.
-void plan_tiles(TileArgs& a, Launch& l, int np_override, int chunk_override)
+void plan_tiles(TileArgs& a, Launch& l, int np_override, int chunk_override, int threads_cap, int prefetch)
 {
     const int elem = lds_elem_bytes(l.dtype);
     const int np_target = np_override > 0 ? np_override : (l.dtype == DT_F64 ? 16 : 32);
@@ -203,7 +209,9 @@ void plan_tiles(TileArgs& a, Launch& l, int np_override, int chunk_override)
     // lane groups per block: best lane utilisation within 1024 threads
     int G = 1, T = 64 * ((jn + 63) / 64);
     double best = double(jn) / T;
-    const int tmax = max_threads(l.op, l.order);
+    if (threads_cap <= 0 && l.op == OP_WENO && l.order == 5) threads_cap = kDefaultWeno5Threads;
+    const int tmax = threads_cap > 0 ? std::min(threads_cap, max_threads(l.op, l.order))
+                                     : max_threads(l.op, l.order);
     for (int g = 2; g <= 16; ++g) {
         const int t = 64 * ((g * jn + 63) / 64);
         if (t > tmax) break;
@@ -229,6 +237,12 @@ void plan_tiles(TileArgs& a, Launch& l, int np_override, int chunk_override)
     a.jn = jn;
     a.threads = T;
     a.slots = static_cast<int>((static_cast<long long>(a.np) * ldj + T - 1) / T);
+    // variant bits: 1 prefetch, 2 coefficients in registers (WENO5), 4 one
+    // tile per workgroup; -1 = automatic
+    const int variant = prefetch < 0 ? kDefaultVariant : prefetch;
+    a.prefetch = variant & 1;
+    l.creg = (variant >> 1) & 1;
+    l.one_tile = (variant >> 2) & 1;
     const long long pen = static_cast<long long>(a.na) * a.nb;
     a.tiles_per_var = static_cast<int>((pen + a.np - 1) / a.np);
     l.blocks_per_cu = 0;
@@ -289,8 +303,7 @@ int validate_reconstruct(const weno_grid* g, int order, int dir, int src_kind, i
 }
 
 int build_reconstruct(const weno_grid* g, int order, int dir, int src_kind, int dtype, const void* src,
-                      const weno_coeffs* c, void* vL, void* vR, int flags, TileArgs& a, Launch& l,
-                      int np_override, int chunk_override)
+                      const weno_coeffs* c, void* vL, void* vR, int flags, TileArgs& a, Launch& l)
 {
     int rc;
     if ((rc = validate_reconstruct(g, order, dir, src_kind, dtype, flags))) return rc;
@@ -329,7 +342,8 @@ int build_reconstruct(const weno_grid* g, int order, int dir, int src_kind, int 
     l.mp = (flags & WENO_FUSE_MP) ? 1 : 0;
     l.dtype = dtype;
     l.nontemporal = (flags & WENO_STREAMING) ? 1 : 0;
-    plan_tiles(a, l, np_override, chunk_override);
+    plan_tiles(a, l, tuned_np(order, dtype, dir), tuned_chunk(order, dtype, dir), tuned_threads(order, dtype, dir),
+               tuned_prefetch(order, dtype, dir));
     return WENO_OK;
 }
 
@@ -638,9 +652,7 @@ int weno_reconstruct(const weno_grid* g, int order, int dir, int src_kind, int d
     int rc;
     TileArgs a;
     Launch l{};
-    if ((rc = build_reconstruct(g, order, dir, src_kind, dtype, src, c, vL, vR, flags, a, l,
-                                tuned_np(order, dtype, dir), tuned_chunk(order, dtype, dir))))
-        return rc;
+    if ((rc = build_reconstruct(g, order, dir, src_kind, dtype, src, c, vL, vR, flags, a, l))) return rc;
     return finish(launch_tile(a, l, stream()), "reconstruct");
 }
 

@@ -41,15 +41,18 @@ int check_dtype(int dtype);
 Dims dims_of(const weno_grid* g, int dir);
 void input_map(const Dims& d, int dir, int src_kind, TileArgs& a);
 void output_map(const Dims& d, int row_len, TileArgs& a);
-void plan_tiles(TileArgs& a, Launch& l, int np_override, int chunk_override);
+void plan_tiles(TileArgs& a, Launch& l, int np_override, int chunk_override, int threads_cap = 0,
+                int prefetch = -1);
 int mode_for(const weno_grid* g, int order);
+int resolve_coeffs(const weno_coeffs* c, Coefs& out, int nc);
 int validate_reconstruct(const weno_grid* g, int order, int dir, int src_kind, int dtype, int flags);
 int build_reconstruct(const weno_grid* g, int order, int dir, int src_kind, int dtype, const void* src,
-                      const weno_coeffs* c, void* vL, void* vR, int flags, TileArgs& a, Launch& l,
-                      int np_override, int chunk_override);
+                      const weno_coeffs* c, void* vL, void* vR, int flags, TileArgs& a, Launch& l);
 
 // tile shapes chosen by the autotuner (0 = default)
 int tuned_np(int order, int dtype, int dir);
 int tuned_chunk(int order, int dtype, int dir);
+int tuned_threads(int order, int dtype, int dir);
+int tuned_prefetch(int order, int dtype, int dir);
 
 }  // namespace wm

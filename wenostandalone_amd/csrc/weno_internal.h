@@ -50,6 +50,7 @@ struct TileArgs {
     int jn;                       // lanes per group (outputs per chunk row)
     int threads;                  // block size
     int slots;                    // tile elements loaded per thread
+    int prefetch;                 // load tile k+1 while computing tile k
     int tiles_per_var;            // pencil groups per variable
     int bpc;                      // blocks per chunk (set at launch)
     int ldj;                      // LDS row stride (elements)
@@ -61,6 +62,8 @@ struct Launch {
     int op, order, mode, mp, dtype;
     int nontemporal;   // streaming stores
     int blocks_per_cu; // 0 = occupancy limit
+    int creg;          // WENO5: beta/d coefficients in registers instead of LDS
+    int one_tile;      // one tile per workgroup (no persistent loop)
 };
 
 int kernel_max_slots();

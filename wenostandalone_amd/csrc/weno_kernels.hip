@@ -27,396 +27,9 @@
 //     If the products leave the normal range the lane falls back to the
 //     per-stencil reciprocals.  TENO keeps the reference's divisions and
 //     powers because its cut-off decision is discontinuous.
-#include "weno_internal.h"
-#include <cfloat>
-#include <cmath>
+#include "weno_device.h"
 
 namespace wm {
-
-struct bf16_t {
-    uint16_t bits;
-};
-
-template <typename TIN>
-struct CT {
-    using type = float;
-};
-template <>
-struct CT<double> {
-    using type = double;
-};
-
-__device__ __forceinline__ double ld_in(const double* p) { return *p; }
-__device__ __forceinline__ float ld_in(const float* p) { return *p; }
-__device__ __forceinline__ float ld_in(const bf16_t* p)
-{
-    return __uint_as_float(static_cast<uint32_t>(p->bits) << 16);
-}
-
-template <typename T, typename V>
-__device__ __forceinline__ void st_out(T* p, V v, bool nt)
-{
-    T x = static_cast<T>(v);
-    if (nt)
-        __builtin_nontemporal_store(x, p);
-    else
-        *p = x;
-}
-
-// Workgroups are dispatched round-robin over the 8 XCDs.  Give each XCD a
-// contiguous range of tiles so that neighbouring tiles (which share boundary
-// cache lines) meet in the same L2.
-__device__ __forceinline__ int xcd_remap(int bid, int nblocks)
-{
-    const int per = nblocks >> 3, rem = nblocks & 7;
-    const int xcd = bid & 7, idx = bid >> 3;
-    return xcd * per + min(xcd, rem) + idx;
-}
-
-// n/d for positive, finite, normal d: hardware reciprocal, two Newton steps
-// and one residual correction (within one ulp of the IEEE quotient).
-__device__ __forceinline__ double fast_div(double n, double d)
-{
-    double r = __builtin_amdgcn_rcp(d);
-    double e = fma(-d, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-d, r, 1.0);
-    r = fma(r, e, r);
-    double q = n * r;
-    double res = fma(-d, q, n);
-    return fma(res, r, q);
-}
-
-// ---------------------------------------------------------------------------
-// per-j coefficients (reference layouts: poly (nc,5,3), d (5,nc), beta (nc,5,6))
-// ---------------------------------------------------------------------------
-template <int ORDER, typename C>
-struct Cf;
-template <typename C>
-struct Cf<3, C> {
-    C pl[2], pr[2];
-};
-template <typename C>
-struct Cf<5, C> {
-    C pl[3][2], pr[3][2];
-};
-template <typename C>
-struct Cf<7, C> {
-    C pl[4][3], pr[4][3];
-};
-
-template <typename C>
-__device__ __forceinline__ void load_cf(const Coefs& c, int jc, Cf<3, C>& f)
-{
-    const int nc = c.nc;
-#pragma unroll
-    for (int r = 0; r < 2; ++r) {
-        f.pl[r] = static_cast<C>(c.polyL[jc + nc * r]);
-        f.pr[r] = static_cast<C>(c.polyR[jc + nc * r]);
-    }
-}
-template <typename C>
-__device__ __forceinline__ void load_cf(const Coefs& c, int jc, Cf<5, C>& f)
-{
-    const int nc = c.nc;
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            f.pl[r][s] = static_cast<C>(c.polyL[jc + nc * (r + 5 * s)]);
-            f.pr[r][s] = static_cast<C>(c.polyR[jc + nc * (r + 5 * s)]);
-        }
-    }
-}
-template <typename C>
-__device__ __forceinline__ void load_cf(const Coefs& c, int jc, Cf<7, C>& f)
-{
-    const int nc = c.nc;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-#pragma unroll
-        for (int s = 0; s < 3; ++s) {
-            f.pl[r][s] = static_cast<C>(c.polyL[jc + nc * (r + 5 * s)]);
-            f.pr[r][s] = static_cast<C>(c.polyR[jc + nc * (r + 5 * s)]);
-        }
-    }
-}
-
-// Smoothness-indicator coefficients beta_coef(j, r, s) and the linear weights
-// d_cbL/d_cbR(r, j) of one lane, kept in an LDS table [entry][lane] shared by
-// the block's lane groups (they hold the same j): entries r*NS+s for beta,
-// then NR entries of dL and NR of dR.  Reading them at use keeps them out of
-// the register budget.
-template <int NS>
-struct BetaRef {
-    const double* p;
-    int stride;
-    int nb;   // number of beta entries (NS*NR)
-    int nr;   // stencils
-    __device__ __forceinline__ double operator()(int r, int s) const { return p[(r * NS + s) * stride]; }
-    __device__ __forceinline__ double dl(int r) const { return p[(nb + r) * stride]; }
-    __device__ __forceinline__ double dr(int r) const { return p[(nb + nr + r) * stride]; }
-};
-template <int ORDER>
-struct BetaTraits {
-    static constexpr int NS = ORDER == 3 ? 1 : (ORDER == 5 ? 3 : 6);   // terms per stencil
-    static constexpr int NR = ORDER == 3 ? 2 : (ORDER == 5 ? 3 : 4);   // stencils
-    static constexpr int N = NS * NR;
-    static constexpr int ENTRIES = N + 2 * NR;
-};
-
-// P_r = prod_{s != r} q_s
-template <int N>
-__device__ __forceinline__ void excl_products(const double (&q)[N], double (&P)[N])
-{
-    if constexpr (N == 2) {
-        P[0] = q[1];
-        P[1] = q[0];
-    } else if constexpr (N == 3) {
-        P[0] = q[1] * q[2];
-        P[1] = q[0] * q[2];
-        P[2] = q[0] * q[1];
-    } else {
-        const double q01 = q[0] * q[1], q23 = q[2] * q[3];
-        P[0] = q[1] * q23;
-        P[1] = q[0] * q23;
-        P[2] = q01 * q[3];
-        P[3] = q01 * q[2];
-    }
-}
-
-// JS: sum_r w_r p_r with w_r = (d_r/q_r) / sum_s (d_s/q_s)
-template <int N, typename C>
-__device__ __forceinline__ double js_combine(const double (&q)[N], const double (&P)[N],
-                                             const double (&d)[N], const C (&p)[N])
-{
-    double S = 0.0, num = 0.0;
-#pragma unroll
-    for (int r = 0; r < N; ++r) {
-        const double a = d[r] * P[r];
-        S += a;
-        num = fma(a, static_cast<double>(p[r]), num);
-    }
-    if (!(S >= DBL_MIN && S <= DBL_MAX)) {
-        S = 0.0;
-        num = 0.0;
-#pragma unroll
-        for (int r = 0; r < N; ++r) {
-            const double a = d[r] / q[r];
-            S += a;
-            num = fma(a, static_cast<double>(p[r]), num);
-        }
-        return num / S;
-    }
-    return fast_div(num, S);
-}
-
-template <int N, typename C>
-__device__ __forceinline__ double masked_combine(const double (&a)[N], const C (&p)[N], double extra)
-{
-    double S = 0.0, num = 0.0;
-#pragma unroll
-    for (int r = 0; r < N; ++r) {
-        S += a[r];
-        num = fma(a[r], static_cast<double>(p[r]), num);
-    }
-    return num / (S + extra);
-}
-
-// ---- WENO3: v[0..2] = j-1, j, j+1 (weno_standalone.f90:392-475) ----------
-template <int MODE, typename C>
-__device__ __forceinline__ void weno3(const C* v, const Cf<3, C>& f, const BetaRef<1>& B, double eps,
-                                      double, double& oL, double& oR)
-{
-    if constexpr (MODE != MODE_JS) {
-        oL = oR = __builtin_nan("");
-        return;
-    } else {
-        const C d0 = v[2] - v[1], dm1 = v[1] - v[0];
-        const C pL[2] = {v[1] + f.pl[0] * d0, v[1] + f.pl[1] * dm1};
-        const C pR[2] = {v[1] + f.pr[0] * d0, v[1] + f.pr[1] * dm1};
-        const double D0 = d0, Dm1 = dm1;
-        const double b0 = B(0, 0) * D0 * D0 + eps;
-        const double b1 = B(1, 0) * Dm1 * Dm1 + eps;
-        const double q[2] = {b0 * b0, b1 * b1};
-        double P[2];
-        excl_products<2>(q, P);
-        const double dl[2] = {B.dl(0), B.dl(1)}, dr[2] = {B.dr(0), B.dr(1)};
-        oL = js_combine<2, C>(q, P, dl, pL);
-        oR = js_combine<2, C>(q, P, dr, pR);
-    }
-}
-
-// ---- WENO5: v[0..4] = j-2..j+2 (weno_standalone.f90:477-669) -------------
-template <int MODE, typename C>
-__device__ __forceinline__ void weno5(const C* v, const Cf<5, C>& f, const BetaRef<3>& B, double eps,
-                                      double ct, double& oL, double& oR)
-{
-    if constexpr (MODE == MODE_NONE) {
-        oL = oR = __builtin_nan("");
-        return;
-    } else {
-        const C vj = v[2];
-        const C dp1 = v[4] - v[3], d0 = v[3] - v[2], dm1 = v[2] - v[1], dm2 = v[1] - v[0];
-        const C pL[3] = {vj + f.pl[0][0] * dp1 + f.pl[0][1] * d0,
-                         vj + f.pl[1][0] * d0 + f.pl[1][1] * dm1,
-                         vj + f.pl[2][0] * dm1 + f.pl[2][1] * dm2};
-        const C pR[3] = {vj + f.pr[0][0] * dp1 + f.pr[0][1] * d0,
-                         vj + f.pr[1][0] * d0 + f.pr[1][1] * dm1,
-                         vj + f.pr[2][0] * dm1 + f.pr[2][1] * dm2};
-        const double Dp1 = dp1, D0 = d0, Dm1 = dm1, Dm2 = dm2;
-        const double b0 = B(0, 0) * Dp1 * Dp1 + B(0, 1) * Dp1 * D0 + B(0, 2) * D0 * D0 + eps;
-        const double b1 = B(1, 0) * D0 * D0 + B(1, 1) * D0 * Dm1 + B(1, 2) * Dm1 * Dm1 + eps;
-        const double b2 = B(2, 0) * Dm1 * Dm1 + B(2, 1) * Dm1 * Dm2 + B(2, 2) * Dm2 * Dm2 + eps;
-        if constexpr (MODE == MODE_JS) {
-            const double q[3] = {b0 * b0, b1 * b1, b2 * b2};
-            double P[3];
-            excl_products<3>(q, P);
-            const double dl[3] = {B.dl(0), B.dl(1), B.dl(2)}, dr[3] = {B.dr(0), B.dr(1), B.dr(2)};
-            oL = js_combine<3, C>(q, P, dl, pL);
-            oR = js_combine<3, C>(q, P, dr, pR);
-        } else {
-            // :578-606, kept with IEEE divisions and pow
-            const double tau = fabs(b2 - b0);
-            const double a0 = pow(1.0 + tau / b0, 6.0);
-            const double a1 = pow(1.0 + tau / b1, 6.0);
-            const double a2 = pow(1.0 + tau / b2, 6.0);
-            const double s = a0 + a1 + a2;
-            const double del0 = (a0 / s < ct) ? 0.0 : 1.0;
-            const double del1 = (a1 / s < ct) ? 0.0 : 1.0;
-            const double del2 = (a2 / s < ct) ? 0.0 : 1.0;
-            const double aL[3] = {del0 * B.dl(0), del1 * B.dl(1), del2 * B.dl(2)};
-            const double aR[3] = {del0 * B.dr(0), del1 * B.dr(1), del2 * B.dr(2)};
-            oL = masked_combine<3, C>(aL, pL, 0.0);
-            oR = masked_combine<3, C>(aR, pR, 0.0);
-        }
-    }
-}
-
-// ---- WENO7: v[0..6] = j-3..j+3 (weno_standalone.f90:1419-1542) -----------
-// poly(q)/beta(q) use coefficient row 3-q, alpha(q) uses d(q): the
-// reference's pairing is kept as is.
-template <int MODE, typename C>
-__device__ __forceinline__ void weno7(const C* v, const Cf<7, C>& f, const BetaRef<6>& B, double eps,
-                                      double ct, double& oL, double& oR)
-{
-    if constexpr (MODE == MODE_NONE) {
-        oL = oR = __builtin_nan("");
-        return;
-    } else {
-        const C vj = v[3];
-        const C d2 = v[6] - v[5], d1 = v[5] - v[4], d0 = v[4] - v[3];
-        const C dm1 = v[3] - v[2], dm2 = v[2] - v[1], dm3 = v[1] - v[0];
-        const C pL[4] = {vj + f.pl[3][0] * dm1 + f.pl[3][1] * dm2 + f.pl[3][2] * dm3,
-                         vj + f.pl[2][0] * d0 + f.pl[2][1] * dm1 + f.pl[2][2] * dm2,
-                         vj + f.pl[1][0] * d1 + f.pl[1][1] * d0 + f.pl[1][2] * dm1,
-                         vj + f.pl[0][0] * d2 + f.pl[0][1] * d1 + f.pl[0][2] * d0};
-        const C pR[4] = {vj + f.pr[3][0] * dm1 + f.pr[3][1] * dm2 + f.pr[3][2] * dm3,
-                         vj + f.pr[2][0] * d0 + f.pr[2][1] * dm1 + f.pr[2][2] * dm2,
-                         vj + f.pr[1][0] * d1 + f.pr[1][1] * d0 + f.pr[1][2] * dm1,
-                         vj + f.pr[0][0] * d2 + f.pr[0][1] * d1 + f.pr[0][2] * d0};
-        const double D2 = d2, D1 = d1, D0 = d0, Dm1 = dm1, Dm2 = dm2, Dm3 = dm3;
-        double b[4];
-        {
-            auto c = [&](int t) { return B(0, t); };
-            b[3] = c(0) * D2 * D2 + c(1) * D2 * D1 + c(2) * D2 * D0 + c(3) * D1 * D1 + c(4) * D1 * D0 +
-                   c(5) * D0 * D0 + eps;
-        }
-        {
-            auto c = [&](int t) { return B(1, t); };
-            b[2] = c(0) * D1 * D1 + c(1) * D1 * D0 + c(2) * D1 * Dm1 + c(3) * D0 * D0 + c(4) * D0 * Dm1 +
-                   c(5) * Dm1 * Dm1 + eps;
-        }
-        {
-            auto c = [&](int t) { return B(2, t); };
-            b[1] = c(0) * D0 * D0 + c(1) * D0 * Dm1 + c(2) * D0 * Dm2 + c(3) * Dm1 * Dm1 +
-                   c(4) * Dm1 * Dm2 + c(5) * Dm2 * Dm2 + eps;
-        }
-        {
-            auto c = [&](int t) { return B(3, t); };
-            b[0] = c(0) * Dm1 * Dm1 + c(1) * Dm1 * Dm2 + c(2) * Dm1 * Dm3 + c(3) * Dm2 * Dm2 +
-                   c(4) * Dm2 * Dm3 + c(5) * Dm3 * Dm3 + eps;
-        }
-        if constexpr (MODE == MODE_JS) {
-            const double q[4] = {b[0] * b[0], b[1] * b[1], b[2] * b[2], b[3] * b[3]};
-            double P[4];
-            excl_products<4>(q, P);
-            const double dl[4] = {B.dl(0), B.dl(1), B.dl(2), B.dl(3)};
-            const double dr[4] = {B.dr(0), B.dr(1), B.dr(2), B.dr(3)};
-            oL = js_combine<4, C>(q, P, dl, pL);
-            oR = js_combine<4, C>(q, P, dr, pR);
-        } else {
-            // :1490-1503: beta(4) keeps weno_eps, alpha(4) is never cut
-            const double tau = fabs(eps - b[3]);
-            double a[5];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const double t = pow(1.0 + tau / b[q], 3.0);
-                a[q] = t * t;
-            }
-            {
-                const double t = pow(1.0 + tau / eps, 3.0);
-                a[4] = t * t;
-            }
-            const double s = a[0] + a[1] + a[2] + a[3] + a[4];
-            double del[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) del[q] = (a[q] / s < ct) ? 0.0 : 1.0;
-            const double aL[4] = {del[0] * B.dl(0), del[1] * B.dl(1), del[2] * B.dl(2), del[3] * B.dl(3)};
-            const double aR[4] = {del[0] * B.dr(0), del[1] * B.dr(1), del[2] * B.dr(2), del[3] * B.dr(3)};
-            oL = masked_combine<4, C>(aL, pL, a[4]);
-            oR = masked_combine<4, C>(aR, pR, a[4]);
-        }
-    }
-}
-
-// ---- monotonicity-preserving limiter (weno_standalone.f90:1544-1615) ------
-// v[0..4] = j-2..j+2.  The two minmod4 values of :1568-1576 are computed once
-// and used in swapped roles for the two sides (:1590-1598).
-template <typename C>
-__device__ __forceinline__ C mp_minmod4(C a, C b)
-{
-    const C one = C(1);
-    const C s4ab = copysign(one, C(4) * a - b);
-    return (s4ab + copysign(one, C(4) * b - a)) *
-           fabs((s4ab + copysign(one, a)) * (s4ab + copysign(one, b))) *
-           fmin(fmin(fmin(fabs(C(4) * a - b), fabs(a)), fabs(C(4) * b - a)), fabs(b)) / C(8);
-}
-
-template <typename C>
-__device__ __forceinline__ C mp_clip(C x, C lo, C hi)
-{
-    const C h = C(0.5);
-    return x + (copysign(h, lo - x) + copysign(h, hi - x)) * fmin(fabs(lo - x), fabs(hi - x));
-}
-
-template <typename C>
-__device__ __forceinline__ void mp_limit(const C* v, C& vL, C& vR)
-{
-    const C vm2 = v[0], vm1 = v[1], v0 = v[2], vp1 = v[3], vp2 = v[4];
-    const C dm1 = v0 + vm2 - C(2) * vm1;
-    const C d0 = vp1 + vm1 - C(2) * v0;
-    const C dp1 = vp2 + v0 - C(2) * vp1;
-    const C m_lo = mp_minmod4(dm1, d0);   // d_MD of the L side, d_LC of the R side
-    const C m_hi = mp_minmod4(d0, dp1);   // d_LC of the L side, d_MD of the R side
-    const C beta_mp = C(4.0 / 3.0);
-    {
-        const C ul = v0 - C(2) * (vp1 - v0);
-        const C md = (v0 + vm1 - m_lo) * C(0.5);
-        const C lc = v0 - (vp1 - v0) * C(0.5) + beta_mp * m_hi;
-        const C lo = fmax(fmin(fmin(v0, vm1), md), fmin(fmin(v0, ul), lc));
-        const C hi = fmin(fmax(fmax(v0, vm1), md), fmax(fmax(v0, ul), lc));
-        vL = mp_clip(vL, lo, hi);
-    }
-    {
-        const C ul = v0 + C(2) * (v0 - vm1);
-        const C md = (v0 + vp1 - m_hi) * C(0.5);
-        const C lc = v0 + (v0 - vm1) * C(0.5) + beta_mp * m_lo;
-        const C lo = fmax(fmin(fmin(v0, vp1), md), fmin(fmin(v0, ul), lc));
-        const C hi = fmin(fmax(fmax(v0, vp1), md), fmax(fmax(v0, ul), lc));
-        vR = mp_clip(vR, lo, hi);
-    }
-}
 
 constexpr int kMaxThreads = 1024;
 constexpr int kMaxNP = 64;
@@ -446,7 +59,7 @@ struct BlockTraits {
 //   registers and is written to the other buffer afterwards: one barrier per
 //   tile.  Pencil address tables are built two tiles ahead (3 slots).
 // ---------------------------------------------------------------------------
-template <int OP, int ORDER, int MODE, bool MP, typename TIN, typename TOUT>
+template <int OP, int ORDER, int MODE, bool MP, typename TIN, typename TOUT, bool CREG>
 __global__ void __launch_bounds__((BlockTraits<OP, ORDER>::kThreads)) sweep_kernel(const TileArgs a, const int nt)
 {
     using C = typename CT<TIN>::type;
@@ -537,13 +150,15 @@ __global__ void __launch_bounds__((BlockTraits<OP, ORDER>::kThreads)) sweep_kern
     }
 
     Cf<OB, C> f;
+    BetaRegs<OB> breg;
     const int jj = tid - gq * a.jn;
     if constexpr (OP == OP_WENO) {
         if (core) {
             int jc = j - 3;
             jc = jc < 0 ? 0 : (jc > a.c.nc - 1 ? a.c.nc - 1 : jc);
             load_cf(a.c, jc, f);
-            if (gq == 0) {
+            if constexpr (CREG) breg.load(a.c, jc);
+            if (!CREG && gq == 0) {
 #pragma unroll
                 for (int t = 0; t < NB; ++t) {
                     const int r = t / NS, s = t - (t / NS) * NS;
@@ -616,7 +231,8 @@ __global__ void __launch_bounds__((BlockTraits<OP, ORDER>::kThreads)) sweep_kern
         C* const nxt = (k & 1) ? lds0 : lds1;
         const int tslot = k % 3;
         const bool more = t + t_stride < t_end;
-        if (more) issue((k + 1) % 3);
+        // prefetch: the next tile's loads are in flight during this compute
+        if (more && a.prefetch) issue((k + 1) % 3);
         if (t + 2 * t_stride < t_end) tables(t + 2 * t_stride, (k + 2) % 3);
 
         if (active) {
@@ -637,19 +253,25 @@ __global__ void __launch_bounds__((BlockTraits<OP, ORDER>::kThreads)) sweep_kern
                     // re-read the LDS coefficient table per pencil: an opaque
                     // pointer keeps the loads from being hoisted into registers
                     const double* bp = beta_lds + jj;
-                    asm volatile("" : "+v"(bp));
-                    const BetaRef<NS> B{bp, a.jn, NB, NR};
+                    if constexpr (!CREG) asm volatile("" : "+v"(bp));
+                    const BetaRef<NS> Bl{bp, a.jn, NB, NR};
                     C v[W];
 #pragma unroll
                     for (int o = 0; o < W; ++o) v[o] = row[o - R];
                     double oL, oR;
                     const C* vc = v + (R - RW);
-                    if constexpr (ORDER == 3)
-                        weno3<MODE, C>(vc, f, B, a.eps, a.teno_ct, oL, oR);
-                    else if constexpr (ORDER == 5)
-                        weno5<MODE, C>(vc, f, B, a.eps, a.teno_ct, oL, oR);
+                    auto cell = [&](const auto& B) {
+                        if constexpr (ORDER == 3)
+                            weno3<MODE, C>(vc, f, B, a.eps, a.teno_ct, oL, oR);
+                        else if constexpr (ORDER == 5)
+                            weno5<MODE, C>(vc, f, B, a.eps, a.teno_ct, oL, oR);
+                        else
+                            weno7<MODE, C>(vc, f, B, a.eps, a.teno_ct, oL, oR);
+                    };
+                    if constexpr (CREG)
+                        cell(breg);
                     else
-                        weno7<MODE, C>(vc, f, B, a.eps, a.teno_ct, oL, oR);
+                        cell(Bl);
                     if constexpr (MP) {
                         C xl = static_cast<C>(oL), xr = static_cast<C>(oR);
                         if (do_mp) mp_limit<C>(v + (R - 2), xl, xr);
@@ -671,8 +293,13 @@ __global__ void __launch_bounds__((BlockTraits<OP, ORDER>::kThreads)) sweep_kern
                 }
             }
         }
-        if (more) commit(nxt);
+        if (more && a.prefetch) commit(nxt);
         __syncthreads();
+        if (more && !a.prefetch) {
+            issue((k + 1) % 3);
+            commit(nxt);
+            __syncthreads();
+        }
     }
 }
 
@@ -691,12 +318,12 @@ static int cu_count()
     return n;
 }
 
-template <int OP, int ORDER, int MODE, bool MP, typename TIN, typename TOUT>
+template <int OP, int ORDER, int MODE, bool MP, typename TIN, typename TOUT, bool CREG = false>
 static hipError_t go(TileArgs a, const Launch& l, hipStream_t st)
 {
     using C = typename CT<TIN>::type;
-    auto kern = sweep_kernel<OP, ORDER, MODE, MP, TIN, TOUT>;
-    constexpr int NB = (OP == OP_WENO) ? BetaTraits<ORDER>::ENTRIES : 0;
+    auto kern = sweep_kernel<OP, ORDER, MODE, MP, TIN, TOUT, CREG>;
+    constexpr int NB = (OP == OP_WENO && !CREG) ? BetaTraits<ORDER>::ENTRIES : 0;
     const size_t lds = ((2 * static_cast<size_t>(a.np) * a.ldj * sizeof(C) + 15) & ~size_t(15)) +
                        static_cast<size_t>(NB) * a.jn * sizeof(double);
     if (lds > 64 * 1024) {
@@ -711,6 +338,7 @@ static hipError_t go(TileArgs a, const Launch& l, hipStream_t st)
     if (l.blocks_per_cu > 0) bpc = cu_count() * l.blocks_per_cu;
     const int need = (ntile + 1) / 2;    // at least two tiles per block
     if (bpc > need) bpc = need;
+    if (l.one_tile) bpc = ntile;         // one tile per workgroup: no persistence
     bpc = (bpc + 7) & ~7;
     a.bpc = bpc;
     hipLaunchKernelGGL(kern, dim3(bpc * a.chunks), dim3(a.threads), lds, st, a, l.nontemporal);
@@ -721,9 +349,13 @@ template <typename TIN, typename TOUT>
 static hipError_t go_weno(const TileArgs& a, const Launch& l, hipStream_t st)
 {
 #define WM_CASE(O, M)                                                             \
-    if (l.order == O && l.mode == M)                                              \
+    if (l.order == O && l.mode == M) {                                            \
+        if (O == 5 && l.creg)                                                     \
+            return l.mp ? go<OP_WENO, O, M, true, TIN, TOUT, true>(a, l, st)      \
+                        : go<OP_WENO, O, M, false, TIN, TOUT, true>(a, l, st);    \
         return l.mp ? go<OP_WENO, O, M, true, TIN, TOUT>(a, l, st)                \
-                    : go<OP_WENO, O, M, false, TIN, TOUT>(a, l, st);
+                    : go<OP_WENO, O, M, false, TIN, TOUT>(a, l, st);              \
+    }
     WM_CASE(3, MODE_NONE)
     WM_CASE(3, MODE_JS)
     WM_CASE(5, MODE_NONE)

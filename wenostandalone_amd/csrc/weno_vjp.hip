@@ -1,0 +1,304 @@
+// Adjoint (vector-Jacobian product) of the WENO5/7-JS reconstruction.
+//
+// For cotangents gL, gR on the core outputs of one sweep, accumulates
+//   gv += (dvL/dv)^T gL + (dvR/dv)^T gR
+// into an array with the input's layout.  Nothing from the forward pass is
+// stored: beta, the weights and the candidate polynomials are recomputed per
+// output.  Per output j (both sides share beta):
+//   vL = sum_r w_r p_r,  w_r = a_r / sum(a),  a_r = d_r / b_r^2
+//   dvL/dp_r = w_r,  dvL/db_r = -2 w_r (p_r - vL) / b_r,
+// p_r is linear and b_r quadratic in the differences D_k = v(J+k+1) - v(J+k).
+// Gather formulation, no atomics: a tile of pencils sits in LDS as in the
+// forward sweep; each lane writes its output's 2R+1 local contributions to
+// LDS, then each input position sums the contributions of the outputs whose
+// stencils cover it; the tile's gradient leaves through the same coalesced
+// slot mapping the forward uses for loads (read-modify-write of gv).
+#include "weno_device.h"
+
+namespace wm {
+
+constexpr int kVjpThreads = 512;
+constexpr int kVjpSlots = 8;
+constexpr int kVjpMaxNP = 32;
+
+// contributions c[0..2R] of one output to v(J-R .. J+R)
+template <int ORDER>
+__device__ __forceinline__ void weno_vjp_cell(const double* v, const Cf<ORDER, double>& f,
+                                              const BetaRef<BetaTraits<ORDER>::NS>& B, double eps, double gl,
+                                              double gr, double* c)
+{
+    constexpr int R = ORDER / 2;
+    constexpr int NR = R + 1;     // candidate stencils
+    constexpr int NT = R;         // polynomial terms per stencil
+    constexpr int ND = 2 * R;     // differences D_k, k = -R..R-1, stored at k+R
+    double D[ND], gD[ND];
+#pragma unroll
+    for (int i = 0; i < ND; ++i) {
+        D[i] = v[i + 1] - v[i];
+        gD[i] = 0.0;
+    }
+    const double vj = v[R];
+    double pL[NR], pR[NR], b[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+        // WENO5: candidate q uses row q and D_{1-q}, D_{-q};
+        // WENO7: candidate q uses row 3-q and D_{q-1}, D_{q-2}, D_{q-3}
+        const int row = (ORDER == 5) ? q : 3 - q;
+        double sL = vj, sR = vj;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int k = (ORDER == 5) ? (1 - q - t) : (q - 1 - t);
+            sL = fma(f.pl[row][t], D[k + R], sL);
+            sR = fma(f.pr[row][t], D[k + R], sR);
+        }
+        pL[q] = sL;
+        pR[q] = sR;
+        if constexpr (ORDER == 5) {
+            const double A = D[1 - q + R], Bv = D[-q + R];
+            b[q] = B(row, 0) * A * A + B(row, 1) * A * Bv + B(row, 2) * Bv * Bv + eps;
+        } else {
+            const double A = D[q - 1 + R], Bv = D[q - 2 + R], Cv = D[q - 3 + R];
+            b[q] = B(row, 0) * A * A + B(row, 1) * A * Bv + B(row, 2) * A * Cv + B(row, 3) * Bv * Bv +
+                   B(row, 4) * Bv * Cv + B(row, 5) * Cv * Cv + eps;
+        }
+    }
+    // normalised weights (shared-product form, per-stencil fallback)
+    double qq[NR], P[NR], wL[NR], wR[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) qq[q] = b[q] * b[q];
+    excl_products<NR>(qq, P);
+    double SL = 0.0, SR = 0.0;
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+        wL[q] = B.dl(q) * P[q];
+        wR[q] = B.dr(q) * P[q];
+        SL += wL[q];
+        SR += wR[q];
+    }
+    if (!(SL >= DBL_MIN && SL <= DBL_MAX && SR >= DBL_MIN && SR <= DBL_MAX)) {
+        SL = SR = 0.0;
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            wL[q] = B.dl(q) / qq[q];
+            wR[q] = B.dr(q) / qq[q];
+            SL += wL[q];
+            SR += wR[q];
+        }
+    }
+    const double iL = fast_div(1.0, SL), iR = fast_div(1.0, SR);
+    double vLo = 0.0, vRo = 0.0;
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+        wL[q] *= iL;
+        wR[q] *= iR;
+        vLo = fma(wL[q], pL[q], vLo);
+        vRo = fma(wR[q], pR[q], vRo);
+    }
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+        const int row = (ORDER == 5) ? q : 3 - q;
+        const double gpL = gl * wL[q], gpR = gr * wR[q];
+        const double gb = -2.0 * fast_div(fma(gpL, pL[q] - vLo, gpR * (pR[q] - vRo)), b[q]);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int k = (ORDER == 5) ? (1 - q - t) : (q - 1 - t);
+            gD[k + R] = fma(gpL, f.pl[row][t], fma(gpR, f.pr[row][t], gD[k + R]));
+        }
+        if constexpr (ORDER == 5) {
+            const int ia = 1 - q + R, ib = -q + R;
+            const double A = D[ia], Bv = D[ib];
+            gD[ia] = fma(gb, 2.0 * B(row, 0) * A + B(row, 1) * Bv, gD[ia]);
+            gD[ib] = fma(gb, B(row, 1) * A + 2.0 * B(row, 2) * Bv, gD[ib]);
+        } else {
+            const int ia = q - 1 + R, ib = q - 2 + R, ic = q - 3 + R;
+            const double A = D[ia], Bv = D[ib], Cv = D[ic];
+            gD[ia] = fma(gb, 2.0 * B(row, 0) * A + B(row, 1) * Bv + B(row, 2) * Cv, gD[ia]);
+            gD[ib] = fma(gb, B(row, 1) * A + 2.0 * B(row, 3) * Bv + B(row, 4) * Cv, gD[ib]);
+            gD[ic] = fma(gb, B(row, 2) * A + B(row, 4) * Bv + 2.0 * B(row, 5) * Cv, gD[ic]);
+        }
+    }
+#pragma unroll
+    for (int o = 0; o <= 2 * R; ++o) c[o] = 0.0;
+    // sum_r w_r = 1 on each side: the centre value's direct derivative
+    c[R] = gl + gr;
+#pragma unroll
+    for (int i = 0; i < ND; ++i) {
+        c[i + 1] += gD[i];
+        c[i] -= gD[i];
+    }
+}
+
+// a.src: v; a.outL/a.outR: gL/gR (read); grad: gv (input layout, accumulated)
+template <int ORDER>
+__global__ void __launch_bounds__(kVjpThreads) vjp_kernel(const TileArgs a, double* __restrict__ grad)
+{
+    constexpr int R = ORDER / 2;
+    constexpr int W = 2 * R + 1;
+    constexpr int NS = BetaTraits<ORDER>::NS;
+    constexpr int NB = BetaTraits<ORDER>::N;
+    constexpr int NR = BetaTraits<ORDER>::NR;
+
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int np = a.np, ldj = a.ldj, G = a.groups, jn = a.jn;
+    double* const tile = reinterpret_cast<double*>(smem_raw);     // [np][ldj] inputs
+    double* const gtile = tile + np * ldj;                         // [np][ldj] gradient
+    double* const cbuf = gtile + np * ldj;                         // [G][W][jn] contributions
+    double* const beta_lds = cbuf + G * W * jn;                    // coefficient table
+    __shared__ long long s_in[kVjpMaxNP];
+    __shared__ long long s_out[kVjpMaxNP];
+
+    const int tid = threadIdx.x, T = blockDim.x;
+    const int ntile = a.tiles_per_var * a.nv;
+    const int bpc = a.bpc;
+    const int xcd = blockIdx.x & 7, rank = blockIdx.x >> 3, nbx = bpc >> 3;
+    const int per = ntile >> 3, rem = ntile & 7;
+    const int t_first = xcd * per + min(xcd, rem) + rank;
+    const int t_end = (xcd + 1) * per + min(xcd + 1, rem);
+    if (t_first >= t_end) return;
+
+    const int c0 = a.j_beg, c1 = a.j_end;            // one chunk: the whole core
+    const int lo = c0 + a.J0f - R, hi = c1 + a.J0f + R;
+    const int nJ = hi - lo + 1;
+    const int gq = tid / jn;
+    const int jj = tid - gq * jn;
+    const int j = c0 + jj;
+    const bool grp = gq < G;
+    const bool core = grp && j <= c1;
+
+    Cf<ORDER, double> f;
+    if (core) {
+        const int jc = j - 3;
+        load_cf(a.c, jc, f);
+        if (gq == 0) {
+            for (int t = 0; t < NB; ++t) {
+                const int r = t / NS, s = t - (t / NS) * NS;
+                beta_lds[t * jn + jj] = a.c.beta[jc + a.c.nc * (r + 5 * s)];
+            }
+            for (int r = 0; r < NR; ++r) {
+                beta_lds[(NB + r) * jn + jj] = a.c.dL[r + 5 * jc];
+                beta_lds[(NB + NR + r) * jn + jj] = a.c.dR[r + 5 * jc];
+            }
+        }
+    }
+    const int npen = a.na * a.nb;
+    const double* const src = static_cast<const double*>(a.src);
+    const double* const gL = static_cast<const double*>(a.outL);
+    const double* const gR = static_cast<const double*>(a.outR);
+
+    for (int t = t_first; t < t_end; t += nbx) {
+        if (tid < np) {
+            const int iv = t / a.tiles_per_var;
+            const int P = (t - iv * a.tiles_per_var) * np + tid;
+            if (P < npen) {
+                const int pa = P % a.na, pb = P / a.na;
+                s_in[tid] = a.src_off + pa * a.s_a + pb * a.s_b + iv * a.s_i + lo * a.s_j;
+                const int k = a.a_is_k ? pa : pb;
+                const int l = a.a_is_k ? pb : pa;
+                s_out[tid] = k * a.o_k + l * a.o_l + iv * a.o_i;
+            } else {
+                s_in[tid] = -1;
+                s_out[tid] = -1;
+            }
+        }
+        __syncthreads();
+        // load the input tile
+        for (int s = 0; s < kVjpSlots; ++s) {
+            const int e = tid + s * T;
+            if (s >= a.slots || e >= np * nJ) break;
+            int q, J;
+            if (a.s_j == 1) {
+                q = e / nJ;
+                J = e - q * nJ;
+            } else {
+                J = e / np;
+                q = e - J * np;
+            }
+            const long long b = s_in[q];
+            tile[q * ldj + J] = b >= 0 ? src[b + J * a.s_j] : 0.0;
+        }
+        __syncthreads();
+        for (int qq = 0; qq < a.npg; ++qq) {
+            const int q = gq * a.npg + qq;
+            double* const cb = cbuf + gq * W * jn;
+            if (core) {
+                const long long ob = s_out[q];
+                double c[W];
+                if (ob >= 0) {
+                    const double* bp = beta_lds + jj;
+                    asm volatile("" : "+v"(bp));
+                    const BetaRef<NS> Bref{bp, jn, NB, NR};
+                    weno_vjp_cell<ORDER>(tile + q * ldj + jj, f, Bref, a.eps, gL[ob + j], gR[ob + j], c);
+                } else {
+#pragma unroll
+                    for (int o = 0; o < W; ++o) c[o] = 0.0;
+                }
+#pragma unroll
+                for (int o = 0; o < W; ++o) cb[o * jn + jj] = c[o];
+            }
+            __syncthreads();
+            if (grp) {
+                const int ncore = c1 - c0 + 1;
+                for (int m = jj; m < nJ; m += jn) {
+                    // position m receives c[o] of output jj' = m - o
+                    double sum = 0.0;
+#pragma unroll
+                    for (int o = 0; o < W; ++o) {
+                        const int js = m - o;
+                        if (js >= 0 && js < ncore) sum += cb[o * jn + js];
+                    }
+                    gtile[q * ldj + m] = sum;
+                }
+            }
+            __syncthreads();
+        }
+        // accumulate the tile's gradient with the coalesced load mapping
+        for (int s = 0; s < kVjpSlots; ++s) {
+            const int e = tid + s * T;
+            if (s >= a.slots || e >= np * nJ) break;
+            int q, J;
+            if (a.s_j == 1) {
+                q = e / nJ;
+                J = e - q * nJ;
+            } else {
+                J = e / np;
+                q = e - J * np;
+            }
+            const long long b = s_in[q];
+            if (b >= 0) grad[b + J * a.s_j] += gtile[q * ldj + J];
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_vjp(TileArgs a, int order, double* grad, hipStream_t st)
+{
+    const size_t lds = (2 * static_cast<size_t>(a.np) * a.ldj + static_cast<size_t>(a.groups) * (order + 0) * a.jn +
+                        static_cast<size_t>(order == 5 ? BetaTraits<5>::ENTRIES : BetaTraits<7>::ENTRIES) * a.jn) *
+                       sizeof(double);
+    const void* kern = order == 5 ? reinterpret_cast<const void*>(vjp_kernel<5>)
+                                  : reinterpret_cast<const void*>(vjp_kernel<7>);
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+        if (e != hipSuccess) return e;
+    }
+    int occ = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, a.threads, lds) != hipSuccess || occ < 1) occ = 1;
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        cus = 256;
+    const int ntile = a.tiles_per_var * a.nv;
+    int bpc = std::min(cus * occ, ntile);
+    bpc = (bpc + 7) & ~7;
+    a.bpc = bpc;
+    if (order == 5)
+        hipLaunchKernelGGL(vjp_kernel<5>, dim3(bpc), dim3(a.threads), lds, st, a, grad);
+    else
+        hipLaunchKernelGGL(vjp_kernel<7>, dim3(bpc), dim3(a.threads), lds, st, a, grad);
+    return hipGetLastError();
+}
+
+int vjp_max_threads() { return kVjpThreads; }
+int vjp_max_slots() { return kVjpSlots; }
+int vjp_max_np() { return kVjpMaxNP; }
+
+}  // namespace wm
