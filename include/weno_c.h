@@ -151,6 +151,46 @@ int weno_mp_limit(const weno_grid* g, int dir, int src_kind, int dtype, const vo
 int weno_reconstruct_vjp(const weno_grid* g, int order, int dir, int src_kind, const void* src,
                          const weno_coeffs* c, const void* gL, const void* gR, void* gv);
 
+/* ---- batched blocks, graphs, autotuning (no reference counterpart) ------ */
+/* One independent block (sub-domain): its own extents, variable count,
+ * order, field, per-direction coefficients and face arrays. */
+typedef struct weno_block {
+    weno_grid grid;
+    int order;                 /* 3, 5 or 7 */
+    const void* v_vf;          /* padded field, FP64 */
+    weno_coeffs coeffs[3];     /* x, y, z */
+    void* vL[3];
+    void* vR[3];
+} weno_block;
+
+/* All blocks' sweeps along dir in one persistent launch (FP64; flags:
+ * WENO_FUSE_MP, WENO_FUSE_COPY).  Work items (block, tile) are pulled from
+ * an atomic counter, largest blocks first. */
+int weno_batched_reconstruct(const weno_block* blocks, int nblocks, int dir, int flags);
+
+/* Captures the x, y and z batched launches into one hipGraph (the blocks'
+ * arrays must stay allocated while the graph lives). */
+int weno_batched_graph_create(const weno_block* blocks, int nblocks, int flags, void** graph);
+
+/* Captures the x, y and z sweeps of weno_reconstruct for one box into one
+ * hipGraph: coeffs3, vL3, vR3 hold the three directions' arguments. */
+int weno_sweep_graph_create(const weno_grid* g, int order, int src_kind, int dtype, const void* src,
+                            const weno_coeffs* coeffs3, void* const* vL3, void* const* vR3, int flags,
+                            void** graph);
+int weno_graph_launch(void* graph);
+int weno_graph_destroy(void* graph);
+
+/* Times tile-shape candidates (pencils per tile x threads x kernel variant)
+ * for the three sweeps of one box and keeps the fastest in the table that
+ * weno_reconstruct consults.  best (optional) receives per direction
+ * {pencils, threads, variant, microseconds}.  Runs the sweeps repeatedly. */
+int weno_autotune_sweep(const weno_grid* g, int order, int dtype, int src_kind, const void* src,
+                        const weno_coeffs* coeffs3, void* const* vL3, void* const* vR3, int flags, int* best);
+
+/* Same for the batched launch: chooses workgroup size and pencils per tile;
+ * best (optional) receives {threads, pencils, microseconds for x+y+z}. */
+int weno_batched_autotune(const weno_block* blocks, int nblocks, int flags, int* best);
+
 /* Plain device-to-device streaming copy (16-byte multiples): the bandwidth
  * yardstick for the sweeps' HBM traffic counters.  No reference counterpart. */
 int weno_stream_copy(void* dst, const void* src, size_t bytes);

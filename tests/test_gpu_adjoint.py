@@ -6,8 +6,10 @@ Checked two independent ways:
     (max|a-b| <= tol * max|b|): each gradient entry sums terms of both signs
     that scale like 1/beta, so entries that nearly cancel carry the rounding
     of the larger terms and an element-wise relative bound is meaningless.
-    tol = 1e-12 on smooth data; 1e-10 on the noisy seeded field, whose 1e-3
-    noise makes beta ~ 1e-6 and the cancelling terms ~1e4 x the result;
+    tol = 1e-11 on smooth data; 1e-10 on the noisy seeded field, whose 1e-3
+    noise makes beta ~ 1e-6 and the cancelling terms ~1e4 x the result
+    (the oracle divides alpha by sum(alpha) per stencil, the GPU uses one
+    shared reciprocal: ulp-level differences in the weights, amplified);
   * against central finite differences (h = 1e-6) of the GPU forward on
     cells of a 16^3 sub-block: 1e-6 relative (FD truncation and roundoff).
 """
@@ -70,7 +72,7 @@ def test_vjp_vs_oracle(order, d, smooth):
     g = wm.make_grid(m, n, p, v=v, buff=b)
     og = oracle.grid(m, n, p, v=v, buff=b)
     f = smooth_field(m, n, p, v, b) if smooth else fields.seeded_field(m, n, p, v, b, seed=60 + d)
-    tol = 1e-12 if smooth else 1e-10
+    tol = 1e-11 if smooth else 1e-10
     nx, ny, nz = fields.extents(m, n, p, b)
     ns, nk, nl = fields.dir_extents(m, n, p, b, d)
     cs = fields.random_coefficients(ns - 6, seed=order + d)

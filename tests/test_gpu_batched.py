@@ -1,0 +1,208 @@
+"""Batched blocks in one persistent launch per direction, hipGraph capture of
+whole x/y/z sweeps, and the tile-shape autotuner.
+
+Every batched output is compared with the oracle sequence of its own block
+(1e-13 relative, bitwise for the order-1 halo); graph replays are compared
+bitwise with the same launches run eagerly, replayed several times with
+eager work and synchronisation in between."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle
+import wenostandalone_amd as wm
+from wenostandalone_amd import fields
+from wenostandalone_amd.device import DeviceArray
+from tests.conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _init():
+    wm.check(wm.lib.weno_init(0))
+    yield
+    for d in range(3):
+        for o in (3, 5, 7):
+            wm.lib.weno_set_tile(o, wm.F64, d, 0, 0, 0, -1)
+
+
+class Blocks:
+    """Host and device data of a set of independent blocks."""
+
+    def __init__(self, specs, seed):
+        self.specs = specs
+        self.keep = []
+        self.blocks = (wm.Block * len(specs))()
+        self.fields, self.coefs, self.faces = [], [], []
+        for b, (m, n, p, v, order) in enumerate(specs):
+            f = fields.seeded_field(m, n, p, v, 6, seed=seed + b)
+            vf = DeviceArray.from_host(f)
+            blk = self.blocks[b]
+            blk.grid = wm.make_grid(m, n, p, v=v, buff=6)
+            blk.order = order
+            blk.v_vf = vf.ptr
+            cs3, fc = [], []
+            for d in range(3):
+                ns, nk, nl = fields.dir_extents(m, n, p, 6, d)
+                cs = fields.random_coefficients(ns - 6, seed=seed * 7 + b * 3 + d)
+                devs = [DeviceArray.from_host(c) for c in cs]
+                blk.coeffs[d] = wm.Coeffs(*[x.ptr for x in devs])
+                L, R = DeviceArray((ns, nk, nl, v)), DeviceArray((ns, nk, nl, v))
+                blk.vL[d], blk.vR[d] = L.ptr, R.ptr
+                self.keep += devs
+                cs3.append(cs)
+                fc.append((L, R))
+            self.keep.append(vf)
+            self.fields.append(f)
+            self.coefs.append(cs3)
+            self.faces.append(fc)
+
+    def fill(self, value):
+        for fc in self.faces:
+            for L, R in fc:
+                L.fill(value)
+                R.fill(-value)
+
+    def download(self):
+        return [[(L.download(), R.download()) for L, R in fc] for fc in self.faces]
+
+    def expected(self, b, d, flags):
+        m, n, p, v, order = self.specs[b]
+        og = oracle.grid(m, n, p, v=v, buff=6)
+        f = self.fields[b]
+        v_rs = oracle.reshape(og, d, f)
+        cL, cR = oracle.order1_copy(og, d, f)
+        eL, eR = oracle.weno(og, order, d, v_rs, self.coefs[b][d], cL, cR)
+        if flags & wm.FUSE_MP:
+            eL, eR = oracle.mp(og, d, v_rs, eL, eR)
+        return eL, eR
+
+
+SPECS = [(9, 14, 6, 2, 5), (17, 8, 11, 3, 7), (6, 6, 6, 1, 5), (13, 19, 10, 2, 3), (21, 9, 15, 2, 7),
+         (7, 12, 18, 3, 5)]
+
+
+@pytest.mark.parametrize("flags", [wm.FUSE_COPY, wm.FUSE_COPY | wm.FUSE_MP])
+def test_batched_vs_oracle(flags):
+    bl = Blocks(SPECS, seed=300)
+    bl.fill(9.5)
+    for d in range(3):
+        wm.check(wm.lib.weno_batched_reconstruct(bl.blocks, len(SPECS), d, flags))
+    got = bl.download()
+    for b in range(len(SPECS)):
+        for d in range(3):
+            eL, eR = bl.expected(b, d, flags)
+            L, R = got[b][d]
+            assert rel_err(L, eL) <= 1e-13, (b, d)
+            assert rel_err(R, eR) <= 1e-13, (b, d)
+
+
+def test_batched_core_only_keeps_halo():
+    bl = Blocks(SPECS[:3], seed=400)
+    bl.fill(9.5)
+    wm.check(wm.lib.weno_batched_reconstruct(bl.blocks, 3, 0, 0))
+    got = bl.download()
+    for b in range(3):
+        L, R = got[b][0]
+        order = SPECS[b][4]
+        if order != 3:
+            assert np.all(L[:3] == 9.5) and np.all(R[-3:] == -9.5)
+        eL, eR = bl.expected(b, 0, 0)
+        s = slice(0, None) if order == 3 else slice(3, -3)
+        assert rel_err(L[s], eL[s]) <= 1e-13 and rel_err(R[s], eR[s]) <= 1e-13
+
+
+def test_batched_graph_replays():
+    bl = Blocks(SPECS, seed=500)
+    flags = wm.FUSE_COPY | wm.FUSE_MP
+    bl.fill(0.0)
+    for d in range(3):
+        wm.check(wm.lib.weno_batched_reconstruct(bl.blocks, len(SPECS), d, flags))
+    eager = bl.download()
+    h = C.c_void_p()
+    wm.check(wm.lib.weno_batched_graph_create(bl.blocks, len(SPECS), flags, C.byref(h)))
+    try:
+        for rep in range(4):
+            bl.fill(float(rep) + 3.0)                       # eager work between replays
+            wm.check(wm.lib.weno_graph_launch(h))
+            wm.check(wm.lib.weno_sync())
+            got = bl.download()
+            for b in range(len(SPECS)):
+                for d in range(3):
+                    assert np.array_equal(got[b][d][0], eager[b][d][0]), (rep, b, d)
+                    assert np.array_equal(got[b][d][1], eager[b][d][1]), (rep, b, d)
+    finally:
+        wm.check(wm.lib.weno_graph_destroy(h))
+
+
+def _box(m, v, seed):
+    g = wm.make_grid(m, v=v, buff=6)
+    f = fields.seeded_field(m, m, m, v, 6, seed=seed)
+    vf = DeviceArray.from_host(f)
+    keep, coefs, L3, R3 = [vf], (wm.Coeffs * 3)(), (C.c_void_p * 3)(), (C.c_void_p * 3)()
+    faces = []
+    for d in range(3):
+        ns, nk, nl = fields.dir_extents(m, m, m, 6, d)
+        devs = [DeviceArray.from_host(c) for c in fields.harness_coefficients(ns - 6, fields.DIR_OFFSETS[d])]
+        keep += devs
+        coefs[d] = wm.Coeffs(*[x.ptr for x in devs])
+        L, R = DeviceArray((ns, nk, nl, v)), DeviceArray((ns, nk, nl, v))
+        faces.append((L, R))
+        L3[d], R3[d] = L.ptr, R.ptr
+    return g, f, vf, keep, coefs, L3, R3, faces
+
+
+@pytest.mark.parametrize("order", [5, 7])
+def test_sweep_graph_replays(order):
+    g, f, vf, keep, coefs, L3, R3, faces = _box(23, 3, seed=9)
+    flags = wm.FUSE_COPY
+    for d in range(3):
+        wm.check(wm.lib.weno_reconstruct(C.byref(g), order, d, wm.SRC_FIELD, wm.F64, vf.vp, C.byref(coefs[d]),
+                                         L3[d], R3[d], flags))
+    eager = [(L.download(), R.download()) for L, R in faces]
+    h = C.c_void_p()
+    wm.check(wm.lib.weno_sweep_graph_create(C.byref(g), order, wm.SRC_FIELD, wm.F64, vf.vp, coefs, L3, R3, flags,
+                                            C.byref(h)))
+    try:
+        for rep in range(3):
+            for L, R in faces:
+                L.fill(-1.0 - rep)
+                R.fill(2.0 + rep)
+            wm.check(wm.lib.weno_graph_launch(h))
+            wm.check(wm.lib.weno_sync())
+            for d, (L, R) in enumerate(faces):
+                assert np.array_equal(L.download(), eager[d][0]) and np.array_equal(R.download(), eager[d][1])
+    finally:
+        wm.check(wm.lib.weno_graph_destroy(h))
+
+
+def test_autotune_sweep_then_parity():
+    g, f, vf, keep, coefs, L3, R3, faces = _box(39, 2, seed=12)
+    best = (C.c_int * 12)()
+    wm.check(wm.lib.weno_autotune_sweep(C.byref(g), 5, wm.F64, wm.SRC_FIELD, vf.vp, coefs, L3, R3,
+                                        wm.FUSE_COPY, best))
+    for d in range(3):
+        assert best[4 * d] > 0 and best[4 * d + 1] >= 64 and best[4 * d + 3] > 0
+    og = oracle.grid(39, v=2, buff=6)
+    for d in range(3):
+        wm.check(wm.lib.weno_reconstruct(C.byref(g), 5, d, wm.SRC_FIELD, wm.F64, vf.vp, C.byref(coefs[d]),
+                                         L3[d], R3[d], wm.FUSE_COPY))
+        ns = fields.dir_extents(39, 39, 39, 6, d)[0]
+        eL, eR = oracle.sequence(og, "weno5", d, f, fields.harness_coefficients(ns - 6, fields.DIR_OFFSETS[d]))
+        assert rel_err(faces[d][0].download(), eL) <= 1e-13 and rel_err(faces[d][1].download(), eR) <= 1e-13
+        wm.check(wm.lib.weno_set_tile(5, wm.F64, d, 0, 0, 0, -1))
+
+
+def test_batched_autotune():
+    bl = Blocks(SPECS, seed=700)
+    best = (C.c_int * 3)()
+    wm.check(wm.lib.weno_batched_autotune(bl.blocks, len(SPECS), wm.FUSE_COPY, best))
+    assert best[0] in (256, 512) and best[1] in (8, 16, 32) and best[2] > 0
+    bl.fill(1.0)
+    wm.check(wm.lib.weno_batched_reconstruct(bl.blocks, len(SPECS), 1, wm.FUSE_COPY))
+    got = bl.download()
+    for b in range(len(SPECS)):
+        eL, eR = bl.expected(b, 1, wm.FUSE_COPY)
+        assert rel_err(got[b][1][0], eL) <= 1e-13 and rel_err(got[b][1][1], eR) <= 1e-13

@@ -45,6 +45,13 @@ class Coeffs(C.Structure):
                 ("dR", C.c_void_p), ("beta", C.c_void_p)]
 
 
+class Block(C.Structure):
+    """weno_block: one independent sub-domain of a batched launch."""
+
+    _fields_ = [("grid", Grid), ("order", C.c_int), ("v_vf", C.c_void_p), ("coeffs", Coeffs * 3),
+                ("vL", C.c_void_p * 3), ("vR", C.c_void_p * 3)]
+
+
 def _load() -> C.CDLL:
     if not os.path.exists(LIB_PATH):
         raise ImportError(
@@ -89,6 +96,15 @@ PROTOTYPES = {
     "weno_set_tile": (_i, [_i, _i, _i, _i, _i, _i, _i]),
     "weno_reconstruct_vjp": (_i, [_G, _i, _i, _i, _vp, _CF, _vp, _vp, _vp]),
     "weno_stream_copy": (_i, [_vp, _vp, _sz]),
+    "weno_batched_reconstruct": (_i, [_vp, _i, _i, _i]),
+    "weno_batched_graph_create": (_i, [_vp, _i, _i, C.POINTER(_vp)]),
+    "weno_sweep_graph_create": (_i, [_G, _i, _i, _i, _vp, _CF, C.POINTER(_vp), C.POINTER(_vp), _i,
+                                     C.POINTER(_vp)]),
+    "weno_graph_launch": (_i, [_vp]),
+    "weno_graph_destroy": (_i, [_vp]),
+    "weno_autotune_sweep": (_i, [_G, _i, _i, _i, _vp, _CF, C.POINTER(_vp), C.POINTER(_vp), _i,
+                                 C.POINTER(_i)]),
+    "weno_batched_autotune": (_i, [_vp, _i, _i, C.POINTER(_i)]),
 }
 
 for _name, (_res, _args) in PROTOTYPES.items():

@@ -4,12 +4,23 @@
 
 namespace wm {
 
+// each workgroup moves one contiguous 16 KiB chunk: four 16-byte loads per
+// lane in flight, then four stores
 __global__ void __launch_bounds__(256) stream_copy_kernel(const double2* __restrict__ src,
                                                           double2* __restrict__ dst, size_t n)
 {
-    size_t i = blockIdx.x * size_t(256) + threadIdx.x;
-    const size_t stride = size_t(gridDim.x) * 256;
-    for (; i < n; i += stride) dst[i] = src[i];
+    const size_t base = blockIdx.x * size_t(1024) + threadIdx.x;
+    double2 r[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const size_t i = base + size_t(u) * 256;
+        if (i < n) r[u] = src[i];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const size_t i = base + size_t(u) * 256;
+        if (i < n) dst[i] = r[u];
+    }
 }
 
 }  // namespace wm
@@ -27,12 +38,9 @@ extern "C" int weno_stream_copy(void* dst, const void* src, size_t bytes)
     if ((reinterpret_cast<uintptr_t>(s) | reinterpret_cast<uintptr_t>(d)) & 15)
         return fail(WENO_E_INVALID, "stream copy needs 16-byte aligned buffers");
     const size_t n = bytes / 16;
-    int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-        cus = 256;
-    size_t blocks = (n + 255) / 256;
-    if (blocks > size_t(cus) * 16) blocks = size_t(cus) * 16;
+    const size_t blocks = (n + 1023) / 1024;
     if (blocks == 0) return WENO_OK;
+    if (blocks > 0x7fffffffu) return fail(WENO_E_INVALID, "stream copy too large");
     hipLaunchKernelGGL(stream_copy_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream(),
                        static_cast<const double2*>(s), static_cast<double2*>(d), n);
     return finish(hipGetLastError(), "stream_copy");
