@@ -31,6 +31,64 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
+# measured HBM traffic of the dominant kernel (TCC FETCH_SIZE / WRITE_SIZE,
+# scaled by the same counters read on a plain copy): written by
+# `bench.py --record-traffic FETCH_CSV WRITE_CSV`, used only while the kernel
+# sources hash the same as when it was measured
+TRAFFIC_RECORD = os.path.join(ROOT, "profiles", "sweep_weno5_fp64_200cube_traffic.json")
+KERNEL_SOURCES = ["weno_kernels.hip", "weno_device.h", "weno_internal.h", "weno_capi.hip", "weno_tune.hip"]
+SWEEP_KERNEL = "sweep_kernel<2, 5, 1, false, double, double"
+
+
+def kernel_source_hash() -> str:
+    import hashlib
+    h = hashlib.sha256()
+    for name in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, "wenostandalone_amd", "csrc", name), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
+
+
+def record_traffic(fetch_csv: str, write_csv: str, m: int = 199, v: int = 8) -> dict:
+    """Per-launch HBM bytes of the x/y/z WENO5 sweeps from two rocprofv3
+    --pmc runs of `bench.py --steps 1 --warmup 0 --no-cpu --copy-yardstick`.
+    The stream copy of one face array (known bytes read and written) gives
+    the KiB-per-count scale of each counter."""
+    import csv
+
+    def rows(path):
+        with open(path) as fh:
+            return [(r["Kernel_Name"], float(r["Counter_Value"])) for r in csv.DictReader(fh)]
+
+    nx = m + 1 + 2 * 6
+    copy_kib = 8 * nx ** 3 * v / 1024
+    fr, wr = rows(fetch_csv), rows(write_csv)
+    cf = [x for k, x in fr if "stream_copy" in k]
+    cw = [x for k, x in wr if "stream_copy" in k]
+    sf = [x for k, x in fr if SWEEP_KERNEL in k]
+    sw = [x for k, x in wr if SWEEP_KERNEL in k]
+    if len(cf) != 1 or len(cw) != 1 or len(sf) != 3 or len(sw) != 3:
+        raise SystemExit("expected one stream copy and three sweeps in each counter file")
+    kf, kw = copy_kib / cf[0], copy_kib / cw[0]
+    per = [(a * kf + b * kw) * 1024 for a, b in zip(sf, sw)]
+    rec = {"kernel_source_sha256": kernel_source_hash(), "traffic_bytes_per_launch": float(np.mean(per)),
+           "per_sweep_bytes": dict(zip("xyz", per)), "fetch_KiB": dict(zip("xyz", sf)),
+           "write_KiB": dict(zip("xyz", sw)), "copy_fetch_KiB": cf[0], "copy_write_KiB": cw[0],
+           "fetch_scale": kf, "write_scale": kw}
+    with open(TRAFFIC_RECORD, "w") as fh:
+        json.dump(rec, fh, indent=1)
+    return rec
+
+
+def recorded_traffic():
+    try:
+        with open(TRAFFIC_RECORD) as fh:
+            rec = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    if rec.get("kernel_source_sha256") != kernel_source_hash():
+        return None
+    return float(rec["traffic_bytes_per_launch"])
 
 
 def hip():
@@ -278,7 +336,12 @@ def main():
     ap.add_argument("--suite", default=None, help="also run the secondary measurements into this JSON file")
     ap.add_argument("--copy-yardstick", action="store_true",
                     help="run one plain streaming copy of a face array first (counter calibration)")
+    ap.add_argument("--record-traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="CPU only: turn two rocprofv3 --pmc counter files into the traffic record")
     args = ap.parse_args()
+    if args.record_traffic:
+        print(json.dumps(record_traffic(*args.record_traffic, m=args.m)))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the sweep would shard "
               "by sub-domain with a buff_size halo exchange; see DESIGN.md).", file=sys.stderr)
@@ -392,9 +455,7 @@ def main():
                      "kernel": "sweep_kernel<OP_WENO,5,JS,noMP,double,double>",
                      "bytes_per_launch": bytes_launch, "avg_launch_ms": launch_ms},
     }
-    tr = os.environ.get("WENO_BENCH_TRAFFIC_BYTES")
-    if tr:
-        out["roofline"]["traffic"] = float(tr)
+    out["roofline"]["traffic"] = recorded_traffic() if m == 199 and args.flags == 0 else None
     for L, R in faces:
         L.free()
         R.free()
