@@ -197,9 +197,12 @@ int weno_stream_copy(void* dst, const void* src, size_t bytes);
 
 /* Tile shape of the sweep kernel for (order, dtype, dir): pencils per tile,
  * outputs per tile along the sweep, threads per workgroup (0 = automatic
- * for each) and prefetch of the next tile during compute (-1 = automatic,
- * 0 off, 1 on).  Used by the autotuner; no reference counterpart. */
-int weno_set_tile(int order, int dtype, int dir, int pencils, int chunk, int threads, int prefetch);
+ * for each) and the kernel variant (-1 = automatic; otherwise bits: 1 load
+ * the next tile during compute, 2 WENO5 coefficients in registers, 4 one
+ * tile per workgroup, 8 keep WENO5-JS FP64 on the general kernel instead of
+ * the producer/consumer kernel, whose tile shape is fixed).  Used by the
+ * autotuner; no reference counterpart. */
+int weno_set_tile(int order, int dtype, int dir, int pencils, int chunk, int threads, int variant);
 
 #ifdef __cplusplus
 }

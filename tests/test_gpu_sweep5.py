@@ -1,0 +1,152 @@
+"""The WENO5-JS FP64 producer/consumer kernel (weno_sweep5.hip).
+
+weno_reconstruct routes WENO5-JS FP64 launches (no fused limiter, both sides,
+rows of >= 48 outputs) to this kernel; the other cases stay on the general
+sweep kernel.  Checked here, 1e-13 relative (rel_err) throughout:
+  * against the oracle on boxes whose three sweeps all take it: field input
+    (contiguous x rows loaded in 16-byte pieces, y/z pencils transposed on the
+    way into LDS) and reshaped input, core-only and fused halo copy (halo
+    bitwise);
+  * rows longer than one tile (ns = 312 along each axis in turn: two chunks);
+  * against the general kernel (variant bit 8) on the same inputs;
+  * fields scaled by 1e22 and 1e-30 (eps 1e-100): the shared denominator
+    (sum_L)(sum_R) over- or underflows, so lanes take the per-stencil IEEE
+    path; isolated 1e22 cells mix both paths inside one wavefront;
+  * NaN input cells: NaN in exactly the outputs whose stencil touches them.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle
+import wenostandalone_amd as wm
+from wenostandalone_amd import fields
+from wenostandalone_amd.device import DeviceArray
+from tests.conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _init():
+    wm.check(wm.lib.weno_init(0))
+    yield
+    for d in range(3):
+        wm.lib.weno_set_tile(5, wm.F64, d, 0, 0, 0, -1)
+
+
+def _coef(cs):
+    devs = [DeviceArray.from_host(c, np.float64) for c in cs]
+    return devs, wm.Coeffs(*[x.ptr for x in devs])
+
+
+def _run(g, d, src, kind, coef, flags, pre=None):
+    ns, nk, nl = fields.dir_extents(g.m_cells, g.n_cells, g.p_cells, g.buff_size, d)
+    vL = DeviceArray((ns, nk, nl, g.v_size))
+    vR = DeviceArray((ns, nk, nl, g.v_size))
+    if pre is None:
+        vL.fill(7.25)
+        vR.fill(-3.5)
+    else:
+        vL.upload(pre[0])
+        vR.upload(pre[1])
+    wm.check(wm.lib.weno_reconstruct(C.byref(g), 5, d, kind, wm.F64, src.vp, C.byref(coef), vL.vp, vR.vp, flags))
+    return vL.download(), vR.download()
+
+
+def _case(m, n, p, v, seed, scale=None, eps=1e-16, nan_cells=()):
+    g = wm.make_grid(m, n, p, v=v, buff=6, eps=eps)
+    og = oracle.grid(m, n, p, v=v, buff=6, eps=eps)
+    f = fields.seeded_field(m, n, p, v, 6, seed=seed)
+    if scale is not None:
+        f = np.asfortranarray(f * scale)
+    for c in nan_cells:
+        f[c] = np.nan
+    return g, og, f
+
+
+def _rel(a, b, split=None):
+    """rel_err, taken separately over outputs of magnitude above and below
+    `split` when a field mixes scales (each part with its own floor)."""
+    if split is None:
+        return rel_err(a, b)
+    big = np.abs(b) > split
+    errs = [0.0]
+    for m in (big, ~big & ~np.isnan(b)):
+        if m.any():
+            errs.append(rel_err(a[m], b[m]))
+    if not np.array_equal(np.isnan(a), np.isnan(b)):
+        return np.inf
+    return max(errs)
+
+
+def _check_box(g, og, f, dirs=(0, 1, 2), seed=0, split=None):
+    m, n, p = g.m_cells, g.n_cells, g.p_cells
+    vf = DeviceArray.from_host(f)
+    for d in dirs:
+        ns = fields.dir_extents(m, n, p, 6, d)[0]
+        cs = fields.random_coefficients(ns - 6, seed=seed + d)
+        devs, coef = _coef(cs)
+        v_rs = oracle.reshape(og, d, f)
+        rs = DeviceArray.from_host(v_rs)
+        cL, cR = oracle.order1_copy(og, d, f)
+        eL, eR = oracle.weno(og, 5, d, v_rs, cs, cL, cR)
+        for src, kind in [(vf, wm.SRC_FIELD), (rs, wm.SRC_RESHAPED)]:
+            L, R = _run(g, d, src, kind, coef, 0, pre=(cL, cR))
+            assert _rel(L, eL, split) <= 1e-13 and _rel(R, eR, split) <= 1e-13, (d, kind, "core")
+            L, R = _run(g, d, src, kind, coef, wm.FUSE_COPY)
+            assert _rel(L, eL, split) <= 1e-13 and _rel(R, eR, split) <= 1e-13, (d, kind, "copy")
+            assert np.array_equal(L[:3], cL[:3], equal_nan=True)
+            assert np.array_equal(R[-3:], cR[-3:], equal_nan=True)
+
+
+def test_all_three_sweeps_vs_oracle():
+    # nx, ny, nz = 62, 68, 74: every sweep has >= 56 core outputs
+    g, og, f = _case(49, 55, 61, 2, seed=501)
+    _check_box(g, og, f, seed=11)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_two_chunk_rows(axis):
+    ext = [7, 9, 8]
+    ext[axis] = 299           # 312 cells along the sweep: two tiles along j
+    g, og, f = _case(*ext, 1, seed=520 + axis)
+    _check_box(g, og, f, dirs=(axis,), seed=30 + axis)
+
+
+def test_matches_general_kernel():
+    g, og, f = _case(49, 55, 61, 2, seed=530)
+    vf = DeviceArray.from_host(f)
+    for d in range(3):
+        ns = fields.dir_extents(49, 55, 61, 6, d)[0]
+        devs, coef = _coef(fields.harness_coefficients(ns - 6, fields.DIR_OFFSETS[d]))
+        fast = _run(g, d, vf, wm.SRC_FIELD, coef, wm.FUSE_COPY)
+        wm.check(wm.lib.weno_set_tile(5, wm.F64, d, 0, 0, 0, 8 | 2))
+        try:
+            gen = _run(g, d, vf, wm.SRC_FIELD, coef, wm.FUSE_COPY)
+        finally:
+            wm.check(wm.lib.weno_set_tile(5, wm.F64, d, 0, 0, 0, -1))
+        assert rel_err(fast[0], gen[0]) <= 1e-13 and rel_err(fast[1], gen[1]) <= 1e-13, d
+
+
+@pytest.mark.parametrize("scale,eps", [(1e22, 1e-16), (1e-30, 1e-100)])
+def test_extreme_scales_take_ieee_path(scale, eps):
+    g, og, f = _case(49, 55, 61, 1, seed=540, scale=scale, eps=eps)
+    _check_box(g, og, f, seed=41)
+
+
+def test_spikes_mix_paths_in_one_wavefront():
+    """Isolated 1e22 cells: lanes whose window holds one take the IEEE path,
+    their neighbours in the same wavefront the shared-reciprocal path."""
+    g, og, f = _case(49, 55, 61, 1, seed=550)
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        f[tuple(int(rng.integers(3, s - 3)) for s in f.shape[:3]) + (0,)] = 1e22
+    _check_box(g, og, f, seed=51, split=1e10)
+
+
+def test_nan_cells_poison_their_stencils_only():
+    cells = [(20, 30, 33, 0), (40, 10, 12, 0), (5, 60, 70, 0)]
+    g, og, f = _case(49, 55, 61, 1, seed=560, nan_cells=cells)
+    _check_box(g, og, f, seed=61)

@@ -243,6 +243,7 @@ void plan_tiles(TileArgs& a, Launch& l, int np_override, int chunk_override, int
     a.prefetch = variant & 1;
     l.creg = (variant >> 1) & 1;
     l.one_tile = (variant >> 2) & 1;
+    l.fast5 = 0;
     const long long pen = static_cast<long long>(a.na) * a.nb;
     a.tiles_per_var = static_cast<int>((pen + a.np - 1) / a.np);
     l.blocks_per_cu = 0;
@@ -342,8 +343,11 @@ int build_reconstruct(const weno_grid* g, int order, int dir, int src_kind, int 
     l.mp = (flags & WENO_FUSE_MP) ? 1 : 0;
     l.dtype = dtype;
     l.nontemporal = (flags & WENO_STREAMING) ? 1 : 0;
+    const int variant = tuned_prefetch(order, dtype, dir);
     plan_tiles(a, l, tuned_np(order, dtype, dir), tuned_chunk(order, dtype, dir), tuned_threads(order, dtype, dir),
-               tuned_prefetch(order, dtype, dir));
+               variant);
+    // variant bit 8 keeps WENO5-JS FP64 on the general kernel
+    if (variant < 0 || !(variant & 8)) l.fast5 = plan_sweep5(a, l, s) + 1;
     return WENO_OK;
 }
 

@@ -1,0 +1,504 @@
+// The FP64 WENO5-JS sweep kernel for gfx950 (the reference's hot path:
+// s_weno5_*_kernel, inputs/weno_standalone.f90:477-1417, in its JS form).
+//
+// One workgroup = 7 (or fewer) consumer waves + 1 producer wave, persistent
+// over a contiguous-per-XCD range of tiles.  A tile is 16 pencils x one
+// j-chunk; tiles flow through two LDS buffers:
+//
+//   producer wave : vector loads of tile k+1 from HBM into registers, then
+//                   into LDS as [pencil][J] (the y/z transposes happen here);
+//                   it never stores to global memory, so its waits on the
+//                   vector memory counter cover its own loads only.
+//   consumer waves: lane = output j (two lane groups of 8 pencils each), the
+//                   per-j coefficients live in registers for the whole launch;
+//                   each lane reads its 5-point window from LDS and writes vL
+//                   and vR straight to HBM.  Consumers never wait on their
+//                   stores.
+//   one barrier per tile hands buffer k+1 to the consumers and buffer k back
+//   to the producer.
+//
+// Per face pair (vL(j), vR(j)) the JS weights are evaluated as
+//     D_i = v(i+1) - v(i),  S_i = D_i^2,  X_i = D_i D_(i-1)
+//     beta_r = B_r0 S + B_r1 X + B_r2 S' + eps              (3 FMA each)
+//     u_r    = prod_{s != r} beta_s^2                     (6 multiplies)
+//     g_r    = (d_r a_r) D + (d_r b_r) D'                 (p_r = v_j + a_r D + b_r D')
+//     out    = v_j + (sum_r u_r g_r) / (sum_r d_r u_r)
+// which is the reference's  sum_r w_r p_r,  w_r = (d_r/beta_r^2)/sum_s(...),
+// regrouped: both sides share one reciprocal of (sum_L)(sum_R).  Lanes whose
+// product leaves the normal range redo the cell with per-stencil IEEE
+// divisions (1/beta_r^2), which also carries NaN and infinity through as the
+// reference does.
+#include "weno_device.h"
+
+#include <algorithm>
+
+namespace wm {
+
+namespace {
+
+constexpr int kNP = 16;     // pencils per tile
+constexpr int kNPG = 8;     // pencils per lane group (two groups)
+constexpr int kMaxThreads5 = 512;
+constexpr int kMaxLoads = 54;   // strided rows: producer registers for 16 pencils x <= 216 J
+constexpr int kRows0 = 224;     // contiguous rows: 14 16-byte pieces x 8 lanes per row
+
+typedef __attribute__((address_space(1))) const double GlobalF64;
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) const f64x2 GlobalF64x2;
+typedef __attribute__((address_space(1))) double GlobalOut;
+
+struct Coef5 {
+    double eL[3], fL[3], eR[3], fR[3];   // d_r * poly(r, 0..1)
+    double b[9];                         // beta_coef(j, r, s), r-major
+    double dL[3], dR[3];
+};
+
+__device__ __forceinline__ void load_coef5(const Coefs& c, int jc, Coef5& k)
+{
+    const int nc = c.nc;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const double dl = c.dL[r + 5 * jc], dr = c.dR[r + 5 * jc];
+        k.dL[r] = dl;
+        k.dR[r] = dr;
+        k.eL[r] = dl * c.polyL[jc + nc * r];
+        k.fL[r] = dl * c.polyL[jc + nc * (r + 5)];
+        k.eR[r] = dr * c.polyR[jc + nc * r];
+        k.fR[r] = dr * c.polyR[jc + nc * (r + 5)];
+#pragma unroll
+        for (int s = 0; s < 3; ++s) k.b[3 * r + s] = c.beta[jc + nc * (r + 5 * s)];
+    }
+}
+
+// positive normal (class bits 8 = +normal)
+__device__ __forceinline__ bool pos_normal(double x) { return __builtin_amdgcn_class(x, 1 << 8); }
+
+// Fast path; returns false when the shared denominator is out of range.
+__device__ __forceinline__ bool cell5(double v0, double v1, double v2, double v3, double v4, const Coef5& k,
+                                      double eps, double& oL, double& oR)
+{
+    const double D3 = v4 - v3, D2 = v3 - v2, D1 = v2 - v1, D0 = v1 - v0;
+    const double S3 = D3 * D3, X3 = D3 * D2, S2 = D2 * D2, X2 = D2 * D1, S1 = D1 * D1, X1 = D1 * D0, S0 = D0 * D0;
+    const double b0 = fma(k.b[0], S3, fma(k.b[1], X3, fma(k.b[2], S2, eps)));
+    const double b1 = fma(k.b[3], S2, fma(k.b[4], X2, fma(k.b[5], S1, eps)));
+    const double b2 = fma(k.b[6], S1, fma(k.b[7], X1, fma(k.b[8], S0, eps)));
+    const double m12 = b1 * b2, m02 = b0 * b2, m01 = b0 * b1;
+    const double u0 = m12 * m12, u1 = m02 * m02, u2 = m01 * m01;
+    const double gL0 = fma(k.eL[0], D3, k.fL[0] * D2), gL1 = fma(k.eL[1], D2, k.fL[1] * D1),
+                 gL2 = fma(k.eL[2], D1, k.fL[2] * D0);
+    const double gR0 = fma(k.eR[0], D3, k.fR[0] * D2), gR1 = fma(k.eR[1], D2, k.fR[1] * D1),
+                 gR2 = fma(k.eR[2], D1, k.fR[2] * D0);
+    const double NL = fma(u0, gL0, fma(u1, gL1, u2 * gL2));
+    const double NR = fma(u0, gR0, fma(u1, gR1, u2 * gR2));
+    const double SL = fma(k.dL[0], u0, fma(k.dL[1], u1, k.dL[2] * u2));
+    const double SR = fma(k.dR[0], u0, fma(k.dR[1], u1, k.dR[2] * u2));
+    const double T = SL * SR;
+    double r = __builtin_amdgcn_rcp(T);
+    double e = fma(-T, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-T, r, 1.0);
+    r = fma(r, e, r);
+    oL = fma(NL, SR * r, v2);
+    oR = fma(NR, SL * r, v2);
+    return pos_normal(T) & pos_normal(r);
+}
+
+// Per-stencil IEEE form (the reference's alpha_r = d_r / beta_r^2).
+__device__ __noinline__ void cell5_slow(double v0, double v1, double v2, double v3, double v4, const Coef5& k,
+                                        double eps, double& oL, double& oR)
+{
+    const double D3 = v4 - v3, D2 = v3 - v2, D1 = v2 - v1, D0 = v1 - v0;
+    const double b0 = fma(k.b[0], D3 * D3, fma(k.b[1], D3 * D2, fma(k.b[2], D2 * D2, eps)));
+    const double b1 = fma(k.b[3], D2 * D2, fma(k.b[4], D2 * D1, fma(k.b[5], D1 * D1, eps)));
+    const double b2 = fma(k.b[6], D1 * D1, fma(k.b[7], D1 * D0, fma(k.b[8], D0 * D0, eps)));
+    const double y0 = 1.0 / (b0 * b0), y1 = 1.0 / (b1 * b1), y2 = 1.0 / (b2 * b2);
+    const double NL = fma(k.eL[0], D3, k.fL[0] * D2) * y0 + fma(k.eL[1], D2, k.fL[1] * D1) * y1 +
+                      fma(k.eL[2], D1, k.fL[2] * D0) * y2;
+    const double NR = fma(k.eR[0], D3, k.fR[0] * D2) * y0 + fma(k.eR[1], D2, k.fR[1] * D1) * y1 +
+                      fma(k.eR[2], D1, k.fR[2] * D0) * y2;
+    const double SL = k.dL[0] * y0 + k.dL[1] * y1 + k.dL[2] * y2;
+    const double SR = k.dR[0] * y0 + k.dR[1] * y1 + k.dR[2] * y2;
+    oL = v2 + NL / SL;
+    oR = v2 + NR / SR;
+}
+
+// Tile t of the launch -> first pencil and variable.
+__device__ __forceinline__ void tile_origin(const TileArgs& a, int t, int& iv, int& P0)
+{
+    iv = t / a.tiles_per_var;
+    P0 = (t - iv * a.tiles_per_var) * kNP;
+}
+
+// Input element offset of pencil P (within variable iv) at J = lo.
+__device__ __forceinline__ long long in_base(const TileArgs& a, int iv, int P, int lo)
+{
+    const int pb = P / a.na, pa = P - pb * a.na;
+    return a.src_off + pa * a.s_a + pb * a.s_b + iv * a.s_i + static_cast<long long>(lo) * a.s_j;
+}
+
+__device__ __forceinline__ long long out_base(const TileArgs& a, int iv, int P)
+{
+    const int pb = P / a.na, pa = P - pb * a.na;
+    const int k = a.a_is_k ? pa : pb, l = a.a_is_k ? pb : pa;
+    return k * a.o_k + l * a.o_l + iv * a.o_i;
+}
+
+}  // namespace
+
+// LAY 0: the sweep runs along the contiguous axis (x of v_vf, or a reshaped
+//        array): each pencil row is loaded with 16-byte loads, 8 lanes per row.
+// LAY 1: the sweep runs along a strided axis (y or z of v_vf): the 16 pencils
+//        of a tile are consecutive along x, loaded as 128-byte row segments
+//        (one pencil per lane, 4 rows per wave instruction) and transposed
+//        into LDS.
+template <int LAY, bool COPY>
+__global__ void __attribute__((amdgpu_flat_work_group_size(64, kMaxThreads5), amdgpu_waves_per_eu(4)))
+sweep5_kernel(const TileArgs a)
+{
+    extern __shared__ __align__(16) double lds[];
+    // per tile and pencil: byte addresses of the pencil's vL and vR rows
+    __shared__ unsigned long long s_ptr[2][kNP][2];
+
+    const int tid = threadIdx.x;
+    const int nwave = blockDim.x >> 6;
+    // wave-uniform role (a scalar branch: each wave runs only its own
+    // path's barriers)
+    const bool producer = __builtin_amdgcn_readfirstlane(tid >> 6) == nwave - 1;
+    const int ldj = a.ldj;
+    double* const buf0 = lds;
+    double* const buf1 = lds + kNP * ldj;
+
+    // ---- chunk and tile range of this workgroup ----
+    const int bpc = a.bpc;
+    const int ch = blockIdx.x / bpc;
+    const int bin = blockIdx.x - ch * bpc;
+    const int ntile = a.tiles_per_var * a.nv;
+    const int xcd = bin & 7, rank = bin >> 3, nbx = bpc >> 3;
+    const int per = ntile >> 3, rem = ntile & 7;
+    const int t_first = xcd * per + min(xcd, rem) + rank;
+    const int t_end = (xcd + 1) * per + min(xcd + 1, rem);
+    if (t_first >= t_end) return;
+    const int npen = a.na * a.nb;
+
+    const int jo0 = ch * a.chunk;
+    const int jo1 = min(a.out_len, jo0 + a.chunk);
+    const int c0 = max(jo0, a.j_beg), c1 = min(jo1 - 1, a.j_end);
+    if (!a.copy_halo && c0 > c1) return;   // nothing to write in this chunk
+    int lo, hi, jlo, jhi;
+    if (a.copy_halo) {
+        lo = jo0 + a.J0f;
+        hi = jo1 - 1 + a.J0f;
+        if (c0 <= c1) {
+            lo = min(lo, c0 + a.J0f - 2);
+            hi = max(hi, c1 + a.J0f + 2);
+        }
+        jlo = jo0;
+        jhi = jo1 - 1;
+    } else {
+        lo = c0 + a.J0f - 2;
+        hi = c1 + a.J0f + 2;
+        jlo = c0;
+        jhi = c1;
+    }
+    // Rows [Lb, Lb + nrow) of every pencil go to LDS.  LAY 1 loads rows in
+    // fours: the window is shifted down (never below 0) so that whole groups
+    // of four rows stay inside the array.  LAY 0 loads 16-byte pieces, so Lb
+    // is even.
+    int Lb, nrow;
+    if (LAY == 0) {
+        Lb = lo & ~1;
+        nrow = (hi - Lb + 2) & ~1;
+    } else {
+        nrow = (hi - lo + 4) & ~3;
+        Lb = min(lo, a.src_len - nrow);
+    }
+
+    if (producer) {
+        const int l = tid & 63;
+        const char* const src = static_cast<const char*>(a.src);
+        f64x2 r2[LAY == 0 ? 2 * 14 : 1];
+        double r1[LAY == 1 ? kMaxLoads : 1];
+        if constexpr (LAY == 0) {
+            // 8 lanes per row (128-byte segments), rows l/8 and l/8 + 8, 14
+            // pieces each (224 elements >= nrow): buffer loads with the
+            // array size as the range, so pieces past the row end read the
+            // next row or, at the very end of the array, zero; they land in
+            // the LDS row padding (ldj = 224) and are never read
+            const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+                const_cast<char*>(src), 0, static_cast<int>(a.src_bytes), 0x00020000);
+            auto fetch = [&](int t) {
+                int iv, P0;
+                tile_origin(a, t, iv, P0);
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int P = min(P0 + (l >> 3) + 8 * h, npen - 1);
+                    const unsigned voff = static_cast<unsigned>(in_base(a, iv, P, Lb) * sizeof(double)) + 16u * (l & 7);
+#pragma unroll
+                    for (int m = 0; m < 14; ++m)
+                        r2[14 * h + m] = __builtin_bit_cast(
+                            f64x2, __builtin_amdgcn_raw_buffer_load_b128(rsrc, voff + 128u * m, 0, 0));
+                }
+            };
+            auto commit = [&](double* buf) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    f64x2* row = reinterpret_cast<f64x2*>(buf + ((l >> 3) + 8 * h) * ldj) + (l & 7);
+#pragma unroll
+                    for (int m = 0; m < 14; ++m) row[8 * m] = r2[14 * h + m];
+                }
+            };
+            auto pointers = [&](int t, int slot) {
+                int iv, P0;
+                tile_origin(a, t, iv, P0);
+                // pencils past the end repeat the last one (same data, same
+                // results, same addresses)
+                if (l < kNP) {
+                    const unsigned long long ob = out_base(a, iv, min(P0 + l, npen - 1)) * sizeof(double);
+                    s_ptr[slot][l][0] = reinterpret_cast<unsigned long long>(a.outL) + ob;
+                    s_ptr[slot][l][1] = reinterpret_cast<unsigned long long>(a.outR) + ob;
+                }
+            };
+            fetch(t_first);
+            commit(buf0);
+            pointers(t_first, 0);
+            __syncthreads();
+            int k = 0;
+            for (int t = t_first; t < t_end; t += nbx, ++k) {
+                const int tn = t + nbx;
+                if (tn < t_end) {
+                    fetch(tn);
+                    commit((k & 1) ? buf0 : buf1);
+                    pointers(tn, (k + 1) & 1);
+                }
+                __syncthreads();
+            }
+        } else {
+            const int nm = nrow >> 2;           // row groups of four
+            const int q = l & 15, J0 = l >> 4;
+            const long long step = 4 * a.s_j * static_cast<long long>(sizeof(double));
+            auto fetch = [&](int t) {
+                int iv, P0;
+                tile_origin(a, t, iv, P0);
+                const long long b0 = in_base(a, iv, P0, Lb);
+                const unsigned lane_off = static_cast<unsigned>(
+                    (in_base(a, iv, min(P0 + q, npen - 1), Lb) - b0 + J0 * a.s_j) * sizeof(double));
+                // uniform row-group addresses on the scalar unit; groups past
+                // the last one re-read it
+                unsigned long long bm = reinterpret_cast<unsigned long long>(src) + b0 * sizeof(double);
+                // per-tile copies: keeps the per-row selects inside the tile
+                // loop instead of hoisted into scalar registers
+                int n = nm;
+                long long st = step;
+                asm volatile("" : "+s"(n), "+s"(st));
+#pragma unroll
+                for (int m = 0; m < kMaxLoads; ++m) {
+                    r1[m] = *reinterpret_cast<const GlobalF64*>(bm + lane_off);
+                    bm += (m + 1 < n) ? st : 0;
+                    asm volatile("" : "+s"(bm));
+                }
+            };
+            auto commit = [&](double* buf) {
+                double* row = buf + q * ldj + J0;
+                int n = nm;
+                asm volatile("" : "+s"(n));
+#pragma unroll
+                for (int m = 0; m < kMaxLoads; ++m) {
+                    *row = r1[m];
+                    row += (m + 1 < n) ? 4 : 0;
+                }
+            };
+            auto pointers = [&](int t, int slot) {
+                int iv, P0;
+                tile_origin(a, t, iv, P0);
+                if (l < kNP) {
+                    const unsigned long long ob = out_base(a, iv, min(P0 + l, npen - 1)) * sizeof(double);
+                    s_ptr[slot][l][0] = reinterpret_cast<unsigned long long>(a.outL) + ob;
+                    s_ptr[slot][l][1] = reinterpret_cast<unsigned long long>(a.outR) + ob;
+                }
+            };
+            fetch(t_first);
+            commit(buf0);
+            pointers(t_first, 0);
+            __syncthreads();
+            int k = 0;
+            for (int t = t_first; t < t_end; t += nbx, ++k) {
+                const int tn = t + nbx;
+                if (tn < t_end) {
+                    fetch(tn);
+                    commit((k & 1) ? buf0 : buf1);
+                    pointers(tn, (k + 1) & 1);
+                }
+                __syncthreads();
+            }
+        }
+        return;
+    }
+
+    // ---- consumer lanes ----
+    const int gq = tid / a.jn;
+    const int j = jlo + (tid - gq * a.jn);
+    const bool active = gq < 2 && j <= jhi;
+    const bool core = active && j >= c0 && j <= c1;
+    const int col = j + a.J0f - Lb;
+    // halo lanes (copy_halo) run the stencil on a clamped window and keep the
+    // copied value
+    const int colc = min(max(col, 2), nrow - 3);
+    Coef5 k5;
+    {
+        int jc = j - 3;
+        jc = jc < 0 ? 0 : (jc > a.c.nc - 1 ? a.c.nc - 1 : jc);
+        load_coef5(a.c, jc, k5);
+    }
+    const double eps = a.eps;
+    const unsigned long long joff = static_cast<unsigned long long>(j) * sizeof(double);
+    const bool halo = COPY && active && !core;
+    const int rbase = gq * kNPG * ldj;
+
+    __syncthreads();
+    int k = 0;
+    for (int t = t_first; t < t_end; t += nbx, ++k) {
+        const double* const cur = ((k & 1) ? buf1 : buf0) + rbase;
+        const int slot = k & 1;
+        if (active) {
+            bool allok = true;
+            int off = colc;
+#pragma unroll
+            for (int qq = 0; qq < kNPG; ++qq) {
+                // one LDS offset per pencil, kept out of registers across
+                // the loop
+                asm volatile("" : "+v"(off));
+                const double* row = cur + off;
+                const double v0 = row[-2], v1 = row[-1], v2 = row[0], v3 = row[1], v4 = row[2];
+                double oL, oR;
+                allok &= cell5(v0, v1, v2, v3, v4, k5, eps, oL, oR);
+                if constexpr (COPY) {
+                    const double cv = row[col - colc];
+                    oL = halo ? cv : oL;
+                    oR = halo ? cv : oR;
+                }
+                const unsigned long long* pp = s_ptr[slot][gq * kNPG + qq];
+                *reinterpret_cast<GlobalOut*>(pp[0] + joff) = oL;
+                *reinterpret_cast<GlobalOut*>(pp[1] + joff) = oR;
+                off += ldj;
+            }
+            if (!allok && !halo) {
+#pragma unroll 1
+                for (int qq = 0; qq < kNPG; ++qq) {
+                    const double* row = cur + qq * ldj + colc;
+                    double oL, oR;
+                    cell5_slow(row[-2], row[-1], row[0], row[1], row[2], k5, eps, oL, oR);
+                    const unsigned long long* pp = s_ptr[slot][gq * kNPG + qq];
+                    *reinterpret_cast<GlobalOut*>(pp[0] + joff) = oL;
+                    *reinterpret_cast<GlobalOut*>(pp[1] + joff) = oR;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Host side: decides whether a WENO5-JS FP64 launch runs on this kernel and
+// sets its tile shape.  Returns the layout (0 or 1), or -1 when the general
+// sweep kernel must be used (other orders/modes/dtypes, the fused limiter,
+// one-sided output, small rows, misaligned contiguous rows).
+int plan_sweep5(TileArgs& a, Launch& l, const void* src)
+{
+    if (l.op != OP_WENO || l.order != 5 || l.mode != MODE_JS || l.mp || l.dtype != DT_F64 || a.sides != 3)
+        return -1;
+    int lay;
+    if (a.s_j == 1) {
+        if ((reinterpret_cast<uintptr_t>(src) & 15) || ((a.src_off | a.s_a | a.s_b | a.s_i) & 1)) return -1;
+        // 32-bit buffer offsets
+        a.src_bytes = static_cast<unsigned long long>(a.s_i) * a.nv * sizeof(double);
+        if (a.src_bytes > 0xffffffffull) return -1;
+        lay = 0;
+    } else if (a.s_a == 1) {
+        if ((a.s_b + kNP + 4 * a.s_j) * static_cast<long long>(sizeof(double)) >= (1LL << 31)) return -1;
+        lay = 1;
+    } else {
+        return -1;
+    }
+    const int max_rows = lay == 0 ? kRows0 : 4 * kMaxLoads;
+    const int ns = a.out_len;
+    for (int nch = 1; nch <= 64; ++nch) {
+        const int chunk = (ns + nch - 1) / nch;
+        int jn = 0, rows = 0;
+        bool fits = true;
+        for (int c = 0; c < nch && fits; ++c) {
+            const int jo0 = c * chunk, jo1 = std::min(ns, jo0 + chunk);
+            if (jo0 >= jo1) break;
+            const int c0 = std::max(jo0, a.j_beg), c1 = std::min(jo1 - 1, a.j_end);
+            int lo, hi, n;
+            if (a.copy_halo) {
+                lo = jo0 + a.J0f;
+                hi = jo1 - 1 + a.J0f;
+                if (c0 <= c1) {
+                    lo = std::min(lo, c0 + a.J0f - 2);
+                    hi = std::max(hi, c1 + a.J0f + 2);
+                }
+                n = jo1 - jo0;
+            } else {
+                if (c0 > c1) continue;
+                lo = c0 + a.J0f - 2;
+                hi = c1 + a.J0f + 2;
+                n = c1 - c0 + 1;
+            }
+            int nrow;
+            if (lay == 0) {
+                nrow = (hi - (lo & ~1) + 2) & ~1;
+                if ((lo & ~1) + nrow > a.src_len + 1) fits = false;   // pieces stay inside the row
+            } else {
+                nrow = (hi - lo + 4) & ~3;
+                if (nrow > a.src_len) fits = false;
+            }
+            if (nrow > max_rows || n > 224) fits = false;
+            jn = std::max(jn, n);
+            rows = std::max(rows, nrow);
+        }
+        if (!fits) continue;
+        if (jn < 48) return -1;   // short rows: the general kernel uses its lanes better
+        a.chunk = chunk;
+        a.chunks = nch;
+        a.jn = jn;
+        a.np = kNP;
+        a.groups = 2;
+        a.npg = kNPG;
+        a.ldj = lay == 0 ? kRows0 : (rows | 1);
+        a.threads = 64 * ((2 * jn + 63) / 64);
+        const long long pen = static_cast<long long>(a.na) * a.nb;
+        a.tiles_per_var = static_cast<int>((pen + kNP - 1) / kNP);
+        return lay;
+    }
+    return -1;
+}
+
+hipError_t launch_sweep5(TileArgs a, int lay, hipStream_t st)
+{
+    const int threads = a.threads + 64;
+    if (threads > kMaxThreads5 || a.ldj > 0xffff) return hipErrorInvalidConfiguration;
+    const size_t lds = 2 * static_cast<size_t>(kNP) * a.ldj * sizeof(double);
+    const bool copy = a.copy_halo != 0;
+    auto kern = lay == 0 ? (copy ? sweep5_kernel<0, true> : sweep5_kernel<0, false>)
+                         : (copy ? sweep5_kernel<1, true> : sweep5_kernel<1, false>);
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+        if (e != hipSuccess) return e;
+    }
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, threads, lds) != hipSuccess || occ < 1) occ = 1;
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    const int ntile = a.tiles_per_var * a.nv;
+    int bpc = cus * occ;
+    const int need = (ntile + 1) / 2;
+    if (bpc > need) bpc = need;
+    bpc = (bpc + 7) & ~7;
+    a.bpc = bpc;
+    hipLaunchKernelGGL(kern, dim3(bpc * a.chunks), dim3(threads), lds, st, a);
+    return hipGetLastError();
+}
+
+}  // namespace wm

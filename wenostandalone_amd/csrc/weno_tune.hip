@@ -62,7 +62,7 @@ extern "C" int weno_set_tile(int order, int dtype, int dir, int pencils, int chu
     if (chunk < 0) return fail(WENO_E_INVALID, "chunk must be >= 0");
     if (threads != 0 && (threads < 64 || threads > 1024 || threads % 64))
         return fail(WENO_E_INVALID, "threads must be 0 or a multiple of 64 in [64,1024] (got %d)", threads);
-    if (prefetch < -1 || prefetch > 7) return fail(WENO_E_INVALID, "variant must be -1 or a bit set in [0,7]");
+    if (prefetch < -1 || prefetch > 15) return fail(WENO_E_INVALID, "variant must be -1 or a bit set in [0,15]");
     set_tuned(order, dtype, dir, pencils, chunk, threads, prefetch);
     return WENO_OK;
 }
