@@ -1,12 +1,13 @@
 """The WENO5-JS FP64 producer/consumer kernel (weno_sweep5.hip).
 
 weno_reconstruct routes WENO5-JS FP64 launches (no fused limiter, both sides,
-rows of >= 48 outputs) to this kernel; the other cases stay on the general
+rows of >= 128 outputs) to this kernel; the other cases stay on the general
 sweep kernel.  Checked here, 1e-13 relative (rel_err) throughout:
   * against the oracle on boxes whose three sweeps all take it: field input
     (contiguous x rows loaded in 16-byte pieces, y/z pencils transposed on the
     way into LDS) and reshaped input, core-only and fused halo copy (halo
-    bitwise);
+    bitwise); core-only launches rewrite the halo cells with the values they
+    hold, so arbitrary halo contents must come back bit for bit;
   * rows longer than one tile (ns = 312 along each axis in turn: two chunks);
   * against the general kernel (variant bit 8) on the same inputs;
   * fields scaled by 1e22 and 1e-30 (eps 1e-100): the shared denominator
@@ -101,10 +102,30 @@ def _check_box(g, og, f, dirs=(0, 1, 2), seed=0, split=None):
             assert np.array_equal(R[-3:], cR[-3:], equal_nan=True)
 
 
+BOX = (139, 131, 127)      # nx, ny, nz = 152, 144, 140: every sweep takes the kernel
+
+
 def test_all_three_sweeps_vs_oracle():
-    # nx, ny, nz = 62, 68, 74: every sweep has >= 56 core outputs
-    g, og, f = _case(49, 55, 61, 2, seed=501)
+    g, og, f = _case(*BOX, 2, seed=501)
     _check_box(g, og, f, seed=11)
+
+
+def test_core_only_keeps_arbitrary_halo():
+    g, og, f = _case(*BOX, 1, seed=505)
+    vf = DeviceArray.from_host(f)
+    rng = np.random.default_rng(9)
+    for d in range(3):
+        ns, nk, nl = fields.dir_extents(*BOX, 6, d)
+        cs = fields.random_coefficients(ns - 6, seed=70 + d)
+        devs, coef = _coef(cs)
+        pL = np.asfortranarray(rng.standard_normal((ns, nk, nl, 1)))
+        pR = np.asfortranarray(rng.standard_normal((ns, nk, nl, 1)))
+        L, R = _run(g, d, vf, wm.SRC_FIELD, coef, 0, pre=(pL, pR))
+        halo = np.r_[0:3, ns - 3:ns]
+        assert np.array_equal(L[halo], pL[halo]) and np.array_equal(R[halo], pR[halo]), d
+        v_rs = oracle.reshape(og, d, f)
+        eL, eR = oracle.weno(og, 5, d, v_rs, cs, pL, pR)
+        assert rel_err(L, eL) <= 1e-13 and rel_err(R, eR) <= 1e-13, d
 
 
 @pytest.mark.parametrize("axis", [0, 1, 2])
@@ -116,10 +137,10 @@ def test_two_chunk_rows(axis):
 
 
 def test_matches_general_kernel():
-    g, og, f = _case(49, 55, 61, 2, seed=530)
+    g, og, f = _case(*BOX, 1, seed=530)
     vf = DeviceArray.from_host(f)
     for d in range(3):
-        ns = fields.dir_extents(49, 55, 61, 6, d)[0]
+        ns = fields.dir_extents(*BOX, 6, d)[0]
         devs, coef = _coef(fields.harness_coefficients(ns - 6, fields.DIR_OFFSETS[d]))
         fast = _run(g, d, vf, wm.SRC_FIELD, coef, wm.FUSE_COPY)
         wm.check(wm.lib.weno_set_tile(5, wm.F64, d, 0, 0, 0, 8 | 2))
@@ -132,14 +153,14 @@ def test_matches_general_kernel():
 
 @pytest.mark.parametrize("scale,eps", [(1e22, 1e-16), (1e-30, 1e-100)])
 def test_extreme_scales_take_ieee_path(scale, eps):
-    g, og, f = _case(49, 55, 61, 1, seed=540, scale=scale, eps=eps)
+    g, og, f = _case(*BOX, 1, seed=540, scale=scale, eps=eps)
     _check_box(g, og, f, seed=41)
 
 
 def test_spikes_mix_paths_in_one_wavefront():
     """Isolated 1e22 cells: lanes whose window holds one take the IEEE path,
     their neighbours in the same wavefront the shared-reciprocal path."""
-    g, og, f = _case(49, 55, 61, 1, seed=550)
+    g, og, f = _case(*BOX, 1, seed=550)
     rng = np.random.default_rng(5)
     for _ in range(200):
         f[tuple(int(rng.integers(3, s - 3)) for s in f.shape[:3]) + (0,)] = 1e22
@@ -147,6 +168,6 @@ def test_spikes_mix_paths_in_one_wavefront():
 
 
 def test_nan_cells_poison_their_stencils_only():
-    cells = [(20, 30, 33, 0), (40, 10, 12, 0), (5, 60, 70, 0)]
-    g, og, f = _case(49, 55, 61, 1, seed=560, nan_cells=cells)
+    cells = [(20, 30, 33, 0), (140, 10, 12, 0), (5, 60, 130, 0)]
+    g, og, f = _case(*BOX, 1, seed=560, nan_cells=cells)
     _check_box(g, og, f, seed=61)

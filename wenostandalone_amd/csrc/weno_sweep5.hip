@@ -1,21 +1,26 @@
 // The FP64 WENO5-JS sweep kernel for gfx950 (the reference's hot path:
 // s_weno5_*_kernel, inputs/weno_standalone.f90:477-1417, in its JS form).
 //
-// One workgroup = 7 (or fewer) consumer waves + 1 producer wave, persistent
-// over a contiguous-per-XCD range of tiles.  A tile is 16 pencils x one
-// j-chunk; tiles flow through two LDS buffers:
+// One workgroup = 7 consumer waves + 1 producer wave, persistent over a
+// contiguous-per-XCD range of tiles.  A tile is 8 pencils x one j-chunk of at
+// most 224 outputs.  Per tile, two phases separated by barriers:
 //
-//   producer wave : vector loads of tile k+1 from HBM into registers, then
-//                   into LDS as [pencil][J] (the y/z transposes happen here);
-//                   it never stores to global memory, so its waits on the
-//                   vector memory counter cover its own loads only.
-//   consumer waves: lane = output j (two lane groups of 8 pencils each), the
-//                   per-j coefficients live in registers for the whole launch;
-//                   each lane reads its 5-point window from LDS and writes vL
-//                   and vR straight to HBM.  Consumers never wait on their
-//                   stores.
-//   one barrier per tile hands buffer k+1 to the consumers and buffer k back
-//   to the producer.
+//   compute: consumer lanes (two groups of one lane per output j, 4 pencils
+//            each; the per-j coefficients live in registers for the whole
+//            launch) read their 5-point windows from the tile's input buffer
+//            in LDS and write vL, vR into an LDS copy of the tile's output
+//            rows.  Meanwhile the producer wave issues the loads of the next
+//            tile (it never stores to global memory, so its waits on the
+//            vector memory counter cover its own loads only).
+//   store:   all consumer waves copy the output rows to HBM, 16 bytes per
+//            lane, whole rows; the producer moves the next tile into the
+//            other input buffer.
+//
+// Output rows are always written whole.  Outputs outside the reconstructed
+// range (j < 3, j > ns-4) get the order-1 copy with WENO_FUSE_COPY; without
+// it they are rewritten with the values they already hold (the producer loads
+// them a tile ahead), which leaves them unchanged: MI355X writes partly
+// filled 128-byte lines back at about 60% of the rate of full ones.
 //
 // Per face pair (vL(j), vR(j)) the JS weights are evaluated as
 //     D_i = v(i+1) - v(i),  S_i = D_i^2,  X_i = D_i D_(i-1)
@@ -36,10 +41,13 @@ namespace wm {
 
 namespace {
 
-constexpr int kNP = 16;     // pencils per tile
-constexpr int kNPG = 8;     // pencils per lane group (two groups)
-constexpr int kMaxThreads5 = 512;
-constexpr int kMaxLoads = 54;   // strided rows: producer registers for 16 pencils x <= 216 J
+constexpr int kNP = 8;          // pencils per tile
+constexpr int kNPG = 4;         // pencils per lane group (two groups)
+constexpr int kConsumers = 448; // 7 waves
+constexpr int kThreads5 = kConsumers + 64;
+constexpr int kMaxOut = 224;    // outputs per chunk row
+constexpr int kRPI = 64 / kNP;  // strided rows: rows per wave instruction (8)
+constexpr int kMaxLoads = 27;   // strided rows: producer registers, 8 x 27 = 216 rows
 constexpr int kRows0 = 224;     // contiguous rows: 14 16-byte pieces x 8 lanes per row
 
 typedef __attribute__((address_space(1))) const double GlobalF64;
@@ -146,17 +154,19 @@ __device__ __forceinline__ long long out_base(const TileArgs& a, int iv, int P)
 }  // namespace
 
 // LAY 0: the sweep runs along the contiguous axis (x of v_vf, or a reshaped
-//        array): each pencil row is loaded with 16-byte loads, 8 lanes per row.
-// LAY 1: the sweep runs along a strided axis (y or z of v_vf): the 16 pencils
-//        of a tile are consecutive along x, loaded as 128-byte row segments
-//        (one pencil per lane, 4 rows per wave instruction) and transposed
+//        array): each pencil row is loaded with 16-byte buffer loads, 8 lanes
+//        per row, 8 rows per wave instruction.
+// LAY 1: the sweep runs along a strided axis (y or z of v_vf): the 8 pencils
+//        of a tile are consecutive along x, loaded as 64-byte row segments
+//        (one pencil per lane, 8 rows per wave instruction) and transposed
 //        into LDS.
 template <int LAY, bool COPY>
-__global__ void __attribute__((amdgpu_flat_work_group_size(64, kMaxThreads5), amdgpu_waves_per_eu(4)))
+__global__ void __attribute__((amdgpu_flat_work_group_size(64, kThreads5), amdgpu_waves_per_eu(4)))
 sweep5_kernel(const TileArgs a)
 {
     extern __shared__ __align__(16) double lds[];
-    // per tile and pencil: byte addresses of the pencil's vL and vR rows
+    // per tile and pencil: byte addresses of the pencil's vL and vR rows at
+    // the chunk's first output
     __shared__ unsigned long long s_ptr[2][kNP][2];
 
     const int tid = threadIdx.x;
@@ -165,8 +175,10 @@ sweep5_kernel(const TileArgs a)
     // path's barriers)
     const bool producer = __builtin_amdgcn_readfirstlane(tid >> 6) == nwave - 1;
     const int ldj = a.ldj;
+    const int ldo = a.ldo;                  // LDS pitch of one output row
     double* const buf0 = lds;
     double* const buf1 = lds + kNP * ldj;
+    double* const obuf = lds + 2 * kNP * ldj;   // [pencil][side][ldo]
 
     // ---- chunk and tile range of this workgroup ----
     const int bpc = a.bpc;
@@ -182,111 +194,71 @@ sweep5_kernel(const TileArgs a)
 
     const int jo0 = ch * a.chunk;
     const int jo1 = min(a.out_len, jo0 + a.chunk);
+    const int nout = jo1 - jo0;             // even (the planner keeps chunks even)
     const int c0 = max(jo0, a.j_beg), c1 = min(jo1 - 1, a.j_end);
-    if (!a.copy_halo && c0 > c1) return;   // nothing to write in this chunk
-    int lo, hi, jlo, jhi;
-    if (a.copy_halo) {
-        lo = jo0 + a.J0f;
-        hi = jo1 - 1 + a.J0f;
-        if (c0 <= c1) {
-            lo = min(lo, c0 + a.J0f - 2);
-            hi = max(hi, c1 + a.J0f + 2);
-        }
-        jlo = jo0;
-        jhi = jo1 - 1;
-    } else {
-        lo = c0 + a.J0f - 2;
-        hi = c1 + a.J0f + 2;
-        jlo = c0;
-        jhi = c1;
+    if (!COPY && c0 > c1) return;   // nothing to reconstruct in this chunk
+    int lo = c0 + a.J0f - 2, hi = c1 + a.J0f + 2;
+    if (COPY) {
+        lo = c0 <= c1 ? min(jo0 + a.J0f, lo) : jo0 + a.J0f;
+        hi = c0 <= c1 ? max(jo1 - 1 + a.J0f, hi) : jo1 - 1 + a.J0f;
     }
     // Rows [Lb, Lb + nrow) of every pencil go to LDS.  LAY 1 loads rows in
-    // fours: the window is shifted down (never below 0) so that whole groups
-    // of four rows stay inside the array.  LAY 0 loads 16-byte pieces, so Lb
+    // groups of kRPI: the window is shifted down (never below 0) so that
+    // whole groups stay inside the array.  LAY 0 loads 16-byte pieces, so Lb
     // is even.
     int Lb, nrow;
     if (LAY == 0) {
         Lb = lo & ~1;
         nrow = (hi - Lb + 2) & ~1;
     } else {
-        nrow = (hi - lo + 4) & ~3;
+        nrow = ((hi - lo + kRPI) / kRPI) * kRPI;
         Lb = min(lo, a.src_len - nrow);
     }
 
     if (producer) {
         const int l = tid & 63;
         const char* const src = static_cast<const char*>(a.src);
-        f64x2 r2[LAY == 0 ? 2 * 14 : 1];
+        f64x2 r2[LAY == 0 ? 14 : 1];
         double r1[LAY == 1 ? kMaxLoads : 1];
-        if constexpr (LAY == 0) {
-            // 8 lanes per row (128-byte segments), rows l/8 and l/8 + 8, 14
-            // pieces each (224 elements >= nrow): buffer loads with the
-            // array size as the range, so pieces past the row end read the
-            // next row or, at the very end of the array, zero; they land in
-            // the LDS row padding (ldj = 224) and are never read
-            const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-                const_cast<char*>(src), 0, static_cast<int>(a.src_bytes), 0x00020000);
-            auto fetch = [&](int t) {
-                int iv, P0;
-                tile_origin(a, t, iv, P0);
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int P = min(P0 + (l >> 3) + 8 * h, npen - 1);
-                    const unsigned voff = static_cast<unsigned>(in_base(a, iv, P, Lb) * sizeof(double)) + 16u * (l & 7);
-#pragma unroll
-                    for (int m = 0; m < 14; ++m)
-                        r2[14 * h + m] = __builtin_bit_cast(
-                            f64x2, __builtin_amdgcn_raw_buffer_load_b128(rsrc, voff + 128u * m, 0, 0));
-                }
-            };
-            auto commit = [&](double* buf) {
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    f64x2* row = reinterpret_cast<f64x2*>(buf + ((l >> 3) + 8 * h) * ldj) + (l & 7);
-#pragma unroll
-                    for (int m = 0; m < 14; ++m) row[8 * m] = r2[14 * h + m];
-                }
-            };
-            auto pointers = [&](int t, int slot) {
-                int iv, P0;
-                tile_origin(a, t, iv, P0);
-                // pencils past the end repeat the last one (same data, same
-                // results, same addresses)
-                if (l < kNP) {
-                    const unsigned long long ob = out_base(a, iv, min(P0 + l, npen - 1)) * sizeof(double);
-                    s_ptr[slot][l][0] = reinterpret_cast<unsigned long long>(a.outL) + ob;
-                    s_ptr[slot][l][1] = reinterpret_cast<unsigned long long>(a.outR) + ob;
-                }
-            };
-            fetch(t_first);
-            commit(buf0);
-            pointers(t_first, 0);
-            __syncthreads();
-            int k = 0;
-            for (int t = t_first; t < t_end; t += nbx, ++k) {
-                const int tn = t + nbx;
-                if (tn < t_end) {
-                    fetch(tn);
-                    commit((k & 1) ? buf0 : buf1);
-                    pointers(tn, (k + 1) & 1);
-                }
-                __syncthreads();
+        // halo cells, one 16-byte piece per lane: pencil l/8, side (l/4)&1,
+        // piece l&3 = outputs {0,1}, {2,3}, {ns-4,ns-3}, {ns-2,ns-1}
+        f64x2 rh = {0.0, 0.0};
+        const int hq = l >> 3, hside = (l >> 2) & 1, hpc = l & 3;
+        const int hj = hpc < 2 ? 2 * hpc : a.out_len - 4 + 2 * (hpc - 2);
+        const bool hin = hj >= jo0 && hj < jo1;
+        auto tile_in = [&](int t) {
+            int iv, P0;
+            tile_origin(a, t, iv, P0);
+            if constexpr (!COPY) {
+                // first: the halo cells of the tile's output rows
+                const unsigned long long base = reinterpret_cast<unsigned long long>(hside ? a.outR : a.outL);
+                const long long ob = out_base(a, iv, min(P0 + hq, npen - 1)) + (hin ? hj : jo0);
+                rh = *reinterpret_cast<const GlobalF64x2*>(base + ob * sizeof(double));
             }
-        } else {
-            const int nm = nrow >> 2;           // row groups of four
-            const int q = l & 15, J0 = l >> 4;
-            const long long step = 4 * a.s_j * static_cast<long long>(sizeof(double));
-            auto fetch = [&](int t) {
-                int iv, P0;
-                tile_origin(a, t, iv, P0);
+            if constexpr (LAY == 0) {
+                // 8 lanes per row (128-byte segments), 14 pieces each
+                // (224 elements >= nrow): buffer loads with the array size as
+                // the range, so pieces past the row end read the next row or,
+                // at the very end of the array, zero; they land in the LDS row
+                // padding (ldj = 224) and are never read
+                const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+                    const_cast<char*>(src), 0, static_cast<int>(a.src_bytes), 0x00020000);
+                const int P = min(P0 + (l >> 3), npen - 1);
+                const unsigned voff = static_cast<unsigned>(in_base(a, iv, P, Lb) * sizeof(double)) + 16u * (l & 7);
+#pragma unroll
+                for (int m = 0; m < 14; ++m)
+                    r2[m] = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(rsrc, voff + 128u * m, 0, 0));
+            } else {
+                const int q = l & 7, J0 = l >> 3;
+                const long long step = kRPI * a.s_j * static_cast<long long>(sizeof(double));
+                const int nm = nrow / kRPI;
                 const long long b0 = in_base(a, iv, P0, Lb);
                 const unsigned lane_off = static_cast<unsigned>(
                     (in_base(a, iv, min(P0 + q, npen - 1), Lb) - b0 + J0 * a.s_j) * sizeof(double));
-                // uniform row-group addresses on the scalar unit; groups past
-                // the last one re-read it
+                // uniform row-group address on the scalar unit; groups past
+                // the last one re-read it.  Per-tile copies of the counts keep
+                // the per-row selects inside the tile loop.
                 unsigned long long bm = reinterpret_cast<unsigned long long>(src) + b0 * sizeof(double);
-                // per-tile copies: keeps the per-row selects inside the tile
-                // loop instead of hoisted into scalar registers
                 int n = nm;
                 long long st = step;
                 asm volatile("" : "+s"(n), "+s"(st));
@@ -296,52 +268,70 @@ sweep5_kernel(const TileArgs a)
                     bm += (m + 1 < n) ? st : 0;
                     asm volatile("" : "+s"(bm));
                 }
-            };
-            auto commit = [&](double* buf) {
-                double* row = buf + q * ldj + J0;
-                int n = nm;
+            }
+        };
+        auto commit = [&](int t, double* buf, int slot) {
+            int iv, P0;
+            tile_origin(a, t, iv, P0);
+            if constexpr (LAY == 0) {
+                f64x2* row = reinterpret_cast<f64x2*>(buf + (l >> 3) * ldj) + (l & 7);
+#pragma unroll
+                for (int m = 0; m < 14; ++m) row[8 * m] = r2[m];
+            } else {
+                double* row = buf + (l & 7) * ldj + (l >> 3);
+                int n = nrow / kRPI;
                 asm volatile("" : "+s"(n));
 #pragma unroll
                 for (int m = 0; m < kMaxLoads; ++m) {
                     *row = r1[m];
-                    row += (m + 1 < n) ? 4 : 0;
+                    row += (m + 1 < n) ? kRPI : 0;
                 }
-            };
-            auto pointers = [&](int t, int slot) {
-                int iv, P0;
-                tile_origin(a, t, iv, P0);
-                if (l < kNP) {
-                    const unsigned long long ob = out_base(a, iv, min(P0 + l, npen - 1)) * sizeof(double);
-                    s_ptr[slot][l][0] = reinterpret_cast<unsigned long long>(a.outL) + ob;
-                    s_ptr[slot][l][1] = reinterpret_cast<unsigned long long>(a.outR) + ob;
-                }
-            };
-            fetch(t_first);
-            commit(buf0);
-            pointers(t_first, 0);
-            __syncthreads();
-            int k = 0;
-            for (int t = t_first; t < t_end; t += nbx, ++k) {
-                const int tn = t + nbx;
-                if (tn < t_end) {
-                    fetch(tn);
-                    commit((k & 1) ? buf0 : buf1);
-                    pointers(tn, (k + 1) & 1);
-                }
-                __syncthreads();
             }
+            // pencils past the end repeat the last one (same data, same
+            // results, same addresses)
+            if (l < kNP) {
+                const unsigned long long ob = (out_base(a, iv, min(P0 + l, npen - 1)) + jo0) * sizeof(double);
+                s_ptr[slot][l][0] = reinterpret_cast<unsigned long long>(a.outL) + ob;
+                s_ptr[slot][l][1] = reinterpret_cast<unsigned long long>(a.outR) + ob;
+            }
+        };
+        // the halo cells held in rh go to the output rows in LDS (core-only
+        // launches); pieces {2,3} and {ns-4,ns-3} carry one reconstructed
+        // output, which is left to the consumers
+        auto put_halo = [&]() {
+            if constexpr (!COPY) {
+                if (hin) {
+                    double* orow = obuf + (hq * 2 + hside) * ldo + (hj - jo0);
+                    if (hpc != 2) orow[0] = rh.x;
+                    if (hpc != 1) orow[1] = rh.y;
+                }
+            }
+        };
+        tile_in(t_first);
+        commit(t_first, buf0, 0);
+        __syncthreads();
+        int k = 0;
+        for (int t = t_first; t < t_end; t += nbx, ++k) {
+            const int tn = t + nbx;
+            // compute phase: halo of tile k into the output rows, loads of
+            // tile k+1 in flight
+            put_halo();
+            if (tn < t_end) tile_in(tn);
+            __syncthreads();
+            // store phase: tile k+1 into the other input buffer
+            if (tn < t_end) commit(tn, (k & 1) ? buf0 : buf1, (k + 1) & 1);
+            __syncthreads();
         }
         return;
     }
 
     // ---- consumer lanes ----
     const int gq = tid / a.jn;
-    const int j = jlo + (tid - gq * a.jn);
-    const bool active = gq < 2 && j <= jhi;
+    const int j = jo0 + (tid - gq * a.jn);
+    const bool active = gq < 2 && j < jo1;
     const bool core = active && j >= c0 && j <= c1;
     const int col = j + a.J0f - Lb;
-    // halo lanes (copy_halo) run the stencil on a clamped window and keep the
-    // copied value
+    // halo lanes run the stencil on a clamped window and keep their value
     const int colc = min(max(col, 2), nrow - 3);
     Coef5 k5;
     {
@@ -350,47 +340,55 @@ sweep5_kernel(const TileArgs a)
         load_coef5(a.c, jc, k5);
     }
     const double eps = a.eps;
-    const unsigned long long joff = static_cast<unsigned long long>(j) * sizeof(double);
-    const bool halo = COPY && active && !core;
-    const int rbase = gq * kNPG * ldj;
+    // a lane writes its output unless it is a halo cell of a core-only launch
+    const bool writes = active && (COPY || core);
+    double* const ocol = obuf + (gq * kNPG * 2) * ldo + (j - jo0);
+    const int nh = (nout / 2 + 63) / 64;     // store instructions per output row
+    const int nseg = 2 * kNP * nh;            // ... per tile
+    const int wave = tid >> 6, lane = tid & 63;
 
     __syncthreads();
     int k = 0;
     for (int t = t_first; t < t_end; t += nbx, ++k) {
-        const double* const cur = ((k & 1) ? buf1 : buf0) + rbase;
+        const double* const cur = ((k & 1) ? buf1 : buf0) + gq * kNPG * ldj;
         const int slot = k & 1;
+        // ---- compute phase ----
         if (active) {
             bool allok = true;
-            int off = colc;
-#pragma unroll
+#pragma unroll 1
             for (int qq = 0; qq < kNPG; ++qq) {
-                // one LDS offset per pencil, kept out of registers across
-                // the loop
-                asm volatile("" : "+v"(off));
-                const double* row = cur + off;
+                const double* row = cur + qq * ldj + colc;
                 const double v0 = row[-2], v1 = row[-1], v2 = row[0], v3 = row[1], v4 = row[2];
                 double oL, oR;
                 allok &= cell5(v0, v1, v2, v3, v4, k5, eps, oL, oR);
                 if constexpr (COPY) {
-                    const double cv = row[col - colc];
-                    oL = halo ? cv : oL;
-                    oR = halo ? cv : oR;
+                    if (!core) oL = oR = cur[qq * ldj + col];
                 }
-                const unsigned long long* pp = s_ptr[slot][gq * kNPG + qq];
-                *reinterpret_cast<GlobalOut*>(pp[0] + joff) = oL;
-                *reinterpret_cast<GlobalOut*>(pp[1] + joff) = oR;
-                off += ldj;
+                if (writes) {
+                    ocol[(2 * qq) * ldo] = oL;
+                    ocol[(2 * qq + 1) * ldo] = oR;
+                }
             }
-            if (!allok && !halo) {
+            if (!allok && core) {
 #pragma unroll 1
                 for (int qq = 0; qq < kNPG; ++qq) {
                     const double* row = cur + qq * ldj + colc;
                     double oL, oR;
                     cell5_slow(row[-2], row[-1], row[0], row[1], row[2], k5, eps, oL, oR);
-                    const unsigned long long* pp = s_ptr[slot][gq * kNPG + qq];
-                    *reinterpret_cast<GlobalOut*>(pp[0] + joff) = oL;
-                    *reinterpret_cast<GlobalOut*>(pp[1] + joff) = oR;
+                    ocol[(2 * qq) * ldo] = oL;
+                    ocol[(2 * qq + 1) * ldo] = oR;
                 }
+            }
+        }
+        __syncthreads();
+        // ---- store phase: whole output rows, 16 bytes per lane ----
+        for (int sgi = wave; sgi < nseg; sgi += kConsumers / 64) {
+            const int row = sgi / nh, part = sgi - row * nh;   // row = pencil * 2 + side
+            const int pc = part * 64 + lane;                    // 16-byte piece of the row
+            if (2 * pc < nout) {
+                const f64x2 v = *reinterpret_cast<const f64x2*>(obuf + row * ldo + 2 * pc);
+                const unsigned long long dst = s_ptr[slot][row >> 1][row & 1] + 16ull * pc;
+                __builtin_nontemporal_store(v, reinterpret_cast<__attribute__((address_space(1))) f64x2*>(dst));
             }
         }
         __syncthreads();
@@ -400,11 +398,15 @@ sweep5_kernel(const TileArgs a)
 // Host side: decides whether a WENO5-JS FP64 launch runs on this kernel and
 // sets its tile shape.  Returns the layout (0 or 1), or -1 when the general
 // sweep kernel must be used (other orders/modes/dtypes, the fused limiter,
-// one-sided output, small rows, misaligned contiguous rows).
+// one-sided output, rows shorter than 96 outputs, odd row lengths or
+// misaligned arrays, inputs too large for 32-bit offsets).
 int plan_sweep5(TileArgs& a, Launch& l, const void* src)
 {
     if (l.op != OP_WENO || l.order != 5 || l.mode != MODE_JS || l.mp || l.dtype != DT_F64 || a.sides != 3)
         return -1;
+    const int ns = a.out_len;
+    if (a.j_beg != 3 || a.j_end != ns - 4 || (ns & 1) || ((a.o_k | a.o_l | a.o_i) & 1)) return -1;
+    if ((reinterpret_cast<uintptr_t>(a.outL) | reinterpret_cast<uintptr_t>(a.outR)) & 15) return -1;
     int lay;
     if (a.s_j == 1) {
         if ((reinterpret_cast<uintptr_t>(src) & 15) || ((a.src_off | a.s_a | a.s_b | a.s_i) & 1)) return -1;
@@ -413,58 +415,47 @@ int plan_sweep5(TileArgs& a, Launch& l, const void* src)
         if (a.src_bytes > 0xffffffffull) return -1;
         lay = 0;
     } else if (a.s_a == 1) {
-        if ((a.s_b + kNP + 4 * a.s_j) * static_cast<long long>(sizeof(double)) >= (1LL << 31)) return -1;
+        if ((a.s_b + kNP + kRPI * a.s_j) * static_cast<long long>(sizeof(double)) >= (1LL << 31)) return -1;
         lay = 1;
     } else {
         return -1;
     }
-    const int max_rows = lay == 0 ? kRows0 : 4 * kMaxLoads;
-    const int ns = a.out_len;
-    for (int nch = 1; nch <= 64; ++nch) {
-        const int chunk = (ns + nch - 1) / nch;
-        int jn = 0, rows = 0;
+    for (int nch = (ns + kMaxOut - 1) / kMaxOut; nch <= 64; ++nch) {
+        const int chunk = ((ns + nch - 1) / nch + 1) & ~1;   // even: 16-byte output pieces
+        int rows = 0;
         bool fits = true;
         for (int c = 0; c < nch && fits; ++c) {
             const int jo0 = c * chunk, jo1 = std::min(ns, jo0 + chunk);
             if (jo0 >= jo1) break;
             const int c0 = std::max(jo0, a.j_beg), c1 = std::min(jo1 - 1, a.j_end);
-            int lo, hi, n;
+            if (c0 > c1 && !a.copy_halo) continue;
+            int lo = c0 + a.J0f - 2, hi = c1 + a.J0f + 2;
             if (a.copy_halo) {
-                lo = jo0 + a.J0f;
-                hi = jo1 - 1 + a.J0f;
-                if (c0 <= c1) {
-                    lo = std::min(lo, c0 + a.J0f - 2);
-                    hi = std::max(hi, c1 + a.J0f + 2);
-                }
-                n = jo1 - jo0;
-            } else {
-                if (c0 > c1) continue;
-                lo = c0 + a.J0f - 2;
-                hi = c1 + a.J0f + 2;
-                n = c1 - c0 + 1;
+                lo = c0 <= c1 ? std::min(jo0 + a.J0f, lo) : jo0 + a.J0f;
+                hi = c0 <= c1 ? std::max(jo1 - 1 + a.J0f, hi) : jo1 - 1 + a.J0f;
             }
             int nrow;
             if (lay == 0) {
                 nrow = (hi - (lo & ~1) + 2) & ~1;
-                if ((lo & ~1) + nrow > a.src_len + 1) fits = false;   // pieces stay inside the row
+                if (nrow > kRows0) fits = false;
             } else {
-                nrow = (hi - lo + 4) & ~3;
-                if (nrow > a.src_len) fits = false;
+                nrow = ((hi - lo + kRPI) / kRPI) * kRPI;
+                if (nrow > a.src_len || nrow > kRPI * kMaxLoads) fits = false;
             }
-            if (nrow > max_rows || n > 224) fits = false;
-            jn = std::max(jn, n);
             rows = std::max(rows, nrow);
         }
+        if (chunk > kMaxOut) fits = false;
         if (!fits) continue;
-        if (jn < 48) return -1;   // short rows: the general kernel uses its lanes better
+        if (chunk < 96) return -1;   // short rows: the general kernel packs more per workgroup
         a.chunk = chunk;
         a.chunks = nch;
-        a.jn = jn;
+        a.jn = chunk;
         a.np = kNP;
         a.groups = 2;
         a.npg = kNPG;
         a.ldj = lay == 0 ? kRows0 : (rows | 1);
-        a.threads = 64 * ((2 * jn + 63) / 64);
+        a.ldo = chunk;
+        a.threads = kThreads5;
         const long long pen = static_cast<long long>(a.na) * a.nb;
         a.tiles_per_var = static_cast<int>((pen + kNP - 1) / kNP);
         return lay;
@@ -474,9 +465,9 @@ int plan_sweep5(TileArgs& a, Launch& l, const void* src)
 
 hipError_t launch_sweep5(TileArgs a, int lay, hipStream_t st)
 {
-    const int threads = a.threads + 64;
-    if (threads > kMaxThreads5 || a.ldj > 0xffff) return hipErrorInvalidConfiguration;
-    const size_t lds = 2 * static_cast<size_t>(kNP) * a.ldj * sizeof(double);
+    if (a.threads != kThreads5 || 2 * a.jn > kConsumers || a.ldj > 0xffff || (a.ldo & 1))
+        return hipErrorInvalidConfiguration;
+    const size_t lds = 2 * static_cast<size_t>(kNP) * (a.ldj + a.ldo) * sizeof(double);
     const bool copy = a.copy_halo != 0;
     auto kern = lay == 0 ? (copy ? sweep5_kernel<0, true> : sweep5_kernel<0, false>)
                          : (copy ? sweep5_kernel<1, true> : sweep5_kernel<1, false>);
@@ -486,7 +477,7 @@ hipError_t launch_sweep5(TileArgs a, int lay, hipStream_t st)
         if (e != hipSuccess) return e;
     }
     int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, threads, lds) != hipSuccess || occ < 1) occ = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, kThreads5, lds) != hipSuccess || occ < 1) occ = 1;
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
@@ -497,7 +488,7 @@ hipError_t launch_sweep5(TileArgs a, int lay, hipStream_t st)
     if (bpc > need) bpc = need;
     bpc = (bpc + 7) & ~7;
     a.bpc = bpc;
-    hipLaunchKernelGGL(kern, dim3(bpc * a.chunks), dim3(threads), lds, st, a);
+    hipLaunchKernelGGL(kern, dim3(bpc * a.chunks), dim3(kThreads5), lds, st, a);
     return hipGetLastError();
 }
 
