@@ -54,7 +54,6 @@ struct TileArgs {
     int tiles_per_var;            // pencil groups per variable
     int bpc;                      // blocks per chunk (set at launch)
     int ldj;                      // LDS row stride (elements)
-    int ldo;                      // LDS output row stride (elements)
     unsigned long long src_bytes; // size of the input array (all variables)
     Coefs c;
     double eps, teno_ct;

@@ -2,24 +2,30 @@
 // s_weno5_*_kernel, inputs/weno_standalone.f90:477-1417, in its JS form).
 //
 // One workgroup = 7 consumer waves + 1 producer wave, persistent over a
-// contiguous-per-XCD range of tiles.  A tile is 8 pencils x one j-chunk of at
-// most 224 outputs.  Per tile, two phases separated by barriers:
+// contiguous-per-XCD range of tiles.  A tile is 16 pencils x one j-chunk of
+// at most 224 outputs; tiles flow through two LDS buffers:
 //
-//   compute: consumer lanes (two groups of one lane per output j, 4 pencils
-//            each; the per-j coefficients live in registers for the whole
-//            launch) read their 5-point windows from the tile's input buffer
-//            in LDS and write vL, vR into an LDS copy of the tile's output
-//            rows.  Meanwhile the producer wave issues the loads of the next
-//            tile (it never stores to global memory, so its waits on the
-//            vector memory counter cover its own loads only).
-//   store:   all consumer waves copy the output rows to HBM, 16 bytes per
-//            lane, whole rows; the producer moves the next tile into the
-//            other input buffer.
+//   producer wave : vector loads of tile k+1 from HBM into registers, then
+//                   into LDS as [pencil][J] (the y/z transposes happen here);
+//                   it never stores to global memory, so its waits on the
+//                   vector memory counter cover its own loads only.
+//   consumer waves: two lane groups, one lane per output j of the chunk
+//                   row; group g takes the tile's pencils 2i+g, so the
+//                   wavefront that holds the end of group 0 and the start of
+//                   group 1 writes the end of one output row and the start
+//                   of the next, which are adjacent in memory for x and y.
+//                   The per-j coefficients live in registers for the whole
+//                   launch; each lane reads its 5-point window from LDS and
+//                   writes vL and vR straight to HBM (non-temporal: the
+//                   outputs are not re-read, the inputs' shared lines are),
+//                   never waiting on its stores.
+//   one barrier per tile hands buffer k+1 to the consumers and buffer k back
+//   to the producer.
 //
 // Output rows are always written whole.  Outputs outside the reconstructed
 // range (j < 3, j > ns-4) get the order-1 copy with WENO_FUSE_COPY; without
 // it they are rewritten with the values they already hold (the producer loads
-// them a tile ahead), which leaves them unchanged: MI355X writes partly
+// them with the tile), which leaves them unchanged: MI355X writes partly
 // filled 128-byte lines back at about 60% of the rate of full ones.
 //
 // Per face pair (vL(j), vR(j)) the JS weights are evaluated as
@@ -41,13 +47,13 @@ namespace wm {
 
 namespace {
 
-constexpr int kNP = 8;          // pencils per tile
-constexpr int kNPG = 4;         // pencils per lane group (two groups)
+constexpr int kNP = 16;         // pencils per tile
+constexpr int kNPG = 8;         // pencils per lane group (two groups)
 constexpr int kConsumers = 448; // 7 waves
 constexpr int kThreads5 = kConsumers + 64;
 constexpr int kMaxOut = 224;    // outputs per chunk row
-constexpr int kRPI = 64 / kNP;  // strided rows: rows per wave instruction (8)
-constexpr int kMaxLoads = 27;   // strided rows: producer registers, 8 x 27 = 216 rows
+constexpr int kRPI = 64 / kNP;  // strided rows: rows per wave instruction (4)
+constexpr int kMaxLoads = 54;   // strided rows: producer registers, 4 x 54 = 216 rows
 constexpr int kRows0 = 224;     // contiguous rows: 14 16-byte pieces x 8 lanes per row
 
 typedef __attribute__((address_space(1))) const double GlobalF64;
@@ -155,19 +161,20 @@ __device__ __forceinline__ long long out_base(const TileArgs& a, int iv, int P)
 
 // LAY 0: the sweep runs along the contiguous axis (x of v_vf, or a reshaped
 //        array): each pencil row is loaded with 16-byte buffer loads, 8 lanes
-//        per row, 8 rows per wave instruction.
-// LAY 1: the sweep runs along a strided axis (y or z of v_vf): the 8 pencils
-//        of a tile are consecutive along x, loaded as 64-byte row segments
-//        (one pencil per lane, 8 rows per wave instruction) and transposed
+//        per row.
+// LAY 1: the sweep runs along a strided axis (y or z of v_vf): the 16 pencils
+//        of a tile are consecutive along x, loaded as 128-byte row segments
+//        (one pencil per lane, 4 rows per wave instruction) and transposed
 //        into LDS.
 template <int LAY, bool COPY>
 __global__ void __attribute__((amdgpu_flat_work_group_size(64, kThreads5), amdgpu_waves_per_eu(4)))
 sweep5_kernel(const TileArgs a)
 {
     extern __shared__ __align__(16) double lds[];
-    // per tile and pencil: byte addresses of the pencil's vL and vR rows at
-    // the chunk's first output
+    // per tile and pencil: byte addresses of the pencil's vL and vR rows
     __shared__ unsigned long long s_ptr[2][kNP][2];
+    // per tile, pencil, halo position and side: the values vL, vR hold there
+    __shared__ double s_halo[2][kNP][6][2];
 
     const int tid = threadIdx.x;
     const int nwave = blockDim.x >> 6;
@@ -175,10 +182,8 @@ sweep5_kernel(const TileArgs a)
     // path's barriers)
     const bool producer = __builtin_amdgcn_readfirstlane(tid >> 6) == nwave - 1;
     const int ldj = a.ldj;
-    const int ldo = a.ldo;                  // LDS pitch of one output row
     double* const buf0 = lds;
     double* const buf1 = lds + kNP * ldj;
-    double* const obuf = lds + 2 * kNP * ldj;   // [pencil][side][ldo]
 
     // ---- chunk and tile range of this workgroup ----
     const int bpc = a.bpc;
@@ -194,7 +199,6 @@ sweep5_kernel(const TileArgs a)
 
     const int jo0 = ch * a.chunk;
     const int jo1 = min(a.out_len, jo0 + a.chunk);
-    const int nout = jo1 - jo0;             // even (the planner keeps chunks even)
     const int c0 = max(jo0, a.j_beg), c1 = min(jo1 - 1, a.j_end);
     if (!COPY && c0 > c1) return;   // nothing to reconstruct in this chunk
     int lo = c0 + a.J0f - 2, hi = c1 + a.J0f + 2;
@@ -218,38 +222,35 @@ sweep5_kernel(const TileArgs a)
     if (producer) {
         const int l = tid & 63;
         const char* const src = static_cast<const char*>(a.src);
-        f64x2 r2[LAY == 0 ? 14 : 1];
+        f64x2 r2[LAY == 0 ? 2 * 14 : 1];
         double r1[LAY == 1 ? kMaxLoads : 1];
-        // halo cells, one 16-byte piece per lane: pencil l/8, side (l/4)&1,
-        // piece l&3 = outputs {0,1}, {2,3}, {ns-4,ns-3}, {ns-2,ns-1}
-        f64x2 rh = {0.0, 0.0};
-        const int hq = l >> 3, hside = (l >> 2) & 1, hpc = l & 3;
-        const int hj = hpc < 2 ? 2 * hpc : a.out_len - 4 + 2 * (hpc - 2);
-        const bool hin = hj >= jo0 && hj < jo1;
+        // halo cells, two 16-byte pieces per lane: idx = l + 64u -> pencil
+        // idx/8, side (idx/4)&1, piece idx&3 = outputs {0,1}, {2,3},
+        // {ns-4,ns-3}, {ns-2,ns-1}
+        f64x2 rh[2] = {{0.0, 0.0}, {0.0, 0.0}};
         auto tile_in = [&](int t) {
             int iv, P0;
             tile_origin(a, t, iv, P0);
-            if constexpr (!COPY) {
-                // first: the halo cells of the tile's output rows
-                const unsigned long long base = reinterpret_cast<unsigned long long>(hside ? a.outR : a.outL);
-                const long long ob = out_base(a, iv, min(P0 + hq, npen - 1)) + (hin ? hj : jo0);
-                rh = *reinterpret_cast<const GlobalF64x2*>(base + ob * sizeof(double));
-            }
             if constexpr (LAY == 0) {
-                // 8 lanes per row (128-byte segments), 14 pieces each
-                // (224 elements >= nrow): buffer loads with the array size as
-                // the range, so pieces past the row end read the next row or,
-                // at the very end of the array, zero; they land in the LDS row
-                // padding (ldj = 224) and are never read
+                // 8 lanes per row (128-byte segments), rows l/8 and l/8 + 8,
+                // 14 pieces each (224 elements >= nrow): buffer loads with the
+                // array size as the range, so pieces past the row end read the
+                // next row or, at the very end of the array, zero; they land
+                // in the LDS row padding (ldj = 224) and are never read
                 const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
                     const_cast<char*>(src), 0, static_cast<int>(a.src_bytes), 0x00020000);
-                const int P = min(P0 + (l >> 3), npen - 1);
-                const unsigned voff = static_cast<unsigned>(in_base(a, iv, P, Lb) * sizeof(double)) + 16u * (l & 7);
 #pragma unroll
-                for (int m = 0; m < 14; ++m)
-                    r2[m] = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(rsrc, voff + 128u * m, 0, 0));
+                for (int h = 0; h < 2; ++h) {
+                    const int P = min(P0 + (l >> 3) + 8 * h, npen - 1);
+                    const unsigned voff =
+                        static_cast<unsigned>(in_base(a, iv, P, Lb) * sizeof(double)) + 16u * (l & 7);
+#pragma unroll
+                    for (int m = 0; m < 14; ++m)
+                        r2[14 * h + m] = __builtin_bit_cast(
+                            f64x2, __builtin_amdgcn_raw_buffer_load_b128(rsrc, voff + 128u * m, 0, 0));
+                }
             } else {
-                const int q = l & 7, J0 = l >> 3;
+                const int q = l & (kNP - 1), J0 = l / kNP;
                 const long long step = kRPI * a.s_j * static_cast<long long>(sizeof(double));
                 const int nm = nrow / kRPI;
                 const long long b0 = in_base(a, iv, P0, Lb);
@@ -269,16 +270,33 @@ sweep5_kernel(const TileArgs a)
                     asm volatile("" : "+s"(bm));
                 }
             }
+            // then the halo cells of the tile's output rows (issued last, so
+            // address arithmetic of the next tile never waits on them)
+            if constexpr (!COPY) {
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const int idx = l + 64 * u;
+                    const int hq = idx >> 3, hside = (idx >> 2) & 1, hpc = idx & 3;
+                    const int hj = hpc < 2 ? 2 * hpc : a.out_len - 4 + 2 * (hpc - 2);
+                    const bool hin = hj >= jo0 && hj < jo1;
+                    const unsigned long long base = reinterpret_cast<unsigned long long>(hside ? a.outR : a.outL);
+                    const long long ob = out_base(a, iv, min(P0 + hq, npen - 1)) + (hin ? hj : jo0);
+                    rh[u] = *reinterpret_cast<const GlobalF64x2*>(base + ob * sizeof(double));
+                }
+            }
         };
         auto commit = [&](int t, double* buf, int slot) {
             int iv, P0;
             tile_origin(a, t, iv, P0);
             if constexpr (LAY == 0) {
-                f64x2* row = reinterpret_cast<f64x2*>(buf + (l >> 3) * ldj) + (l & 7);
 #pragma unroll
-                for (int m = 0; m < 14; ++m) row[8 * m] = r2[m];
+                for (int h = 0; h < 2; ++h) {
+                    f64x2* row = reinterpret_cast<f64x2*>(buf + ((l >> 3) + 8 * h) * ldj) + (l & 7);
+#pragma unroll
+                    for (int m = 0; m < 14; ++m) row[8 * m] = r2[14 * h + m];
+                }
             } else {
-                double* row = buf + (l & 7) * ldj + (l >> 3);
+                double* row = buf + (l & (kNP - 1)) * ldj + l / kNP;
                 int n = nrow / kRPI;
                 asm volatile("" : "+s"(n));
 #pragma unroll
@@ -287,24 +305,23 @@ sweep5_kernel(const TileArgs a)
                     row += (m + 1 < n) ? kRPI : 0;
                 }
             }
+            if constexpr (!COPY) {
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const int idx = l + 64 * u;
+                    const int hq = idx >> 3, hside = (idx >> 2) & 1, hpc = idx & 3;
+                    // positions 0-2: outputs 0-2, 3-5: ns-3..ns-1; pieces {2,3}
+                    // and {ns-4,ns-3} carry one reconstructed output each
+                    if (hpc != 2) s_halo[slot][hq][hpc < 2 ? 2 * hpc : 2 * hpc - 2][hside] = rh[u].x;
+                    if (hpc != 1) s_halo[slot][hq][hpc < 2 ? 2 * hpc + 1 : 2 * hpc - 1][hside] = rh[u].y;
+                }
+            }
             // pencils past the end repeat the last one (same data, same
             // results, same addresses)
             if (l < kNP) {
-                const unsigned long long ob = (out_base(a, iv, min(P0 + l, npen - 1)) + jo0) * sizeof(double);
+                const unsigned long long ob = out_base(a, iv, min(P0 + l, npen - 1)) * sizeof(double);
                 s_ptr[slot][l][0] = reinterpret_cast<unsigned long long>(a.outL) + ob;
                 s_ptr[slot][l][1] = reinterpret_cast<unsigned long long>(a.outR) + ob;
-            }
-        };
-        // the halo cells held in rh go to the output rows in LDS (core-only
-        // launches); pieces {2,3} and {ns-4,ns-3} carry one reconstructed
-        // output, which is left to the consumers
-        auto put_halo = [&]() {
-            if constexpr (!COPY) {
-                if (hin) {
-                    double* orow = obuf + (hq * 2 + hside) * ldo + (hj - jo0);
-                    if (hpc != 2) orow[0] = rh.x;
-                    if (hpc != 1) orow[1] = rh.y;
-                }
             }
         };
         tile_in(t_first);
@@ -313,13 +330,10 @@ sweep5_kernel(const TileArgs a)
         int k = 0;
         for (int t = t_first; t < t_end; t += nbx, ++k) {
             const int tn = t + nbx;
-            // compute phase: halo of tile k into the output rows, loads of
-            // tile k+1 in flight
-            put_halo();
-            if (tn < t_end) tile_in(tn);
-            __syncthreads();
-            // store phase: tile k+1 into the other input buffer
-            if (tn < t_end) commit(tn, (k & 1) ? buf0 : buf1, (k + 1) & 1);
+            if (tn < t_end) {
+                tile_in(tn);
+                commit(tn, (k & 1) ? buf0 : buf1, (k + 1) & 1);
+            }
             __syncthreads();
         }
         return;
@@ -330,9 +344,11 @@ sweep5_kernel(const TileArgs a)
     const int j = jo0 + (tid - gq * a.jn);
     const bool active = gq < 2 && j < jo1;
     const bool core = active && j >= c0 && j <= c1;
+    const bool halo = active && !core;
     const int col = j + a.J0f - Lb;
     // halo lanes run the stencil on a clamped window and keep their value
     const int colc = min(max(col, 2), nrow - 3);
+    const int hpos = j < a.j_beg ? j - (a.j_beg - 3) : j - (a.j_end + 1) + 3;
     Coef5 k5;
     {
         int jc = j - 3;
@@ -340,55 +356,46 @@ sweep5_kernel(const TileArgs a)
         load_coef5(a.c, jc, k5);
     }
     const double eps = a.eps;
-    // a lane writes its output unless it is a halo cell of a core-only launch
-    const bool writes = active && (COPY || core);
-    double* const ocol = obuf + (gq * kNPG * 2) * ldo + (j - jo0);
-    const int nh = (nout / 2 + 63) / 64;     // store instructions per output row
-    const int nseg = 2 * kNP * nh;            // ... per tile
-    const int wave = tid >> 6, lane = tid & 63;
+    const unsigned long long joff = static_cast<unsigned long long>(j) * sizeof(double);
 
     __syncthreads();
     int k = 0;
     for (int t = t_first; t < t_end; t += nbx, ++k) {
-        const double* const cur = ((k & 1) ? buf1 : buf0) + gq * kNPG * ldj;
         const int slot = k & 1;
-        // ---- compute phase ----
+        // group g: pencils g, g+2, g+4, ...
+        const double* const cur = ((k & 1) ? buf1 : buf0) + gq * ldj;
         if (active) {
             bool allok = true;
 #pragma unroll 1
             for (int qq = 0; qq < kNPG; ++qq) {
-                const double* row = cur + qq * ldj + colc;
+                const int q = 2 * qq + gq;
+                const double* row = cur + 2 * qq * ldj + colc;
                 const double v0 = row[-2], v1 = row[-1], v2 = row[0], v3 = row[1], v4 = row[2];
                 double oL, oR;
                 allok &= cell5(v0, v1, v2, v3, v4, k5, eps, oL, oR);
-                if constexpr (COPY) {
-                    if (!core) oL = oR = cur[qq * ldj + col];
+                if (halo) {
+                    if constexpr (COPY) {
+                        oL = oR = row[col - colc];
+                    } else {
+                        oL = s_halo[slot][q][hpos][0];
+                        oR = s_halo[slot][q][hpos][1];
+                    }
                 }
-                if (writes) {
-                    ocol[(2 * qq) * ldo] = oL;
-                    ocol[(2 * qq + 1) * ldo] = oR;
-                }
+                const unsigned long long* pp = s_ptr[slot][q];
+                __builtin_nontemporal_store(oL, reinterpret_cast<GlobalOut*>(pp[0] + joff));
+                __builtin_nontemporal_store(oR, reinterpret_cast<GlobalOut*>(pp[1] + joff));
             }
             if (!allok && core) {
 #pragma unroll 1
                 for (int qq = 0; qq < kNPG; ++qq) {
-                    const double* row = cur + qq * ldj + colc;
+                    const int q = 2 * qq + gq;
+                    const double* row = cur + 2 * qq * ldj + colc;
                     double oL, oR;
                     cell5_slow(row[-2], row[-1], row[0], row[1], row[2], k5, eps, oL, oR);
-                    ocol[(2 * qq) * ldo] = oL;
-                    ocol[(2 * qq + 1) * ldo] = oR;
+                    const unsigned long long* pp = s_ptr[slot][q];
+                    __builtin_nontemporal_store(oL, reinterpret_cast<GlobalOut*>(pp[0] + joff));
+                    __builtin_nontemporal_store(oR, reinterpret_cast<GlobalOut*>(pp[1] + joff));
                 }
-            }
-        }
-        __syncthreads();
-        // ---- store phase: whole output rows, 16 bytes per lane ----
-        for (int sgi = wave; sgi < nseg; sgi += kConsumers / 64) {
-            const int row = sgi / nh, part = sgi - row * nh;   // row = pencil * 2 + side
-            const int pc = part * 64 + lane;                    // 16-byte piece of the row
-            if (2 * pc < nout) {
-                const f64x2 v = *reinterpret_cast<const f64x2*>(obuf + row * ldo + 2 * pc);
-                const unsigned long long dst = s_ptr[slot][row >> 1][row & 1] + 16ull * pc;
-                __builtin_nontemporal_store(v, reinterpret_cast<__attribute__((address_space(1))) f64x2*>(dst));
             }
         }
         __syncthreads();
@@ -446,7 +453,7 @@ int plan_sweep5(TileArgs& a, Launch& l, const void* src)
         }
         if (chunk > kMaxOut) fits = false;
         if (!fits) continue;
-        if (chunk < 96) return -1;   // short rows: the general kernel packs more per workgroup
+        if (chunk < 96) return -1;   // short rows: the general kernel uses its lanes better
         a.chunk = chunk;
         a.chunks = nch;
         a.jn = chunk;
@@ -454,7 +461,6 @@ int plan_sweep5(TileArgs& a, Launch& l, const void* src)
         a.groups = 2;
         a.npg = kNPG;
         a.ldj = lay == 0 ? kRows0 : (rows | 1);
-        a.ldo = chunk;
         a.threads = kThreads5;
         const long long pen = static_cast<long long>(a.na) * a.nb;
         a.tiles_per_var = static_cast<int>((pen + kNP - 1) / kNP);
@@ -465,9 +471,8 @@ int plan_sweep5(TileArgs& a, Launch& l, const void* src)
 
 hipError_t launch_sweep5(TileArgs a, int lay, hipStream_t st)
 {
-    if (a.threads != kThreads5 || 2 * a.jn > kConsumers || a.ldj > 0xffff || (a.ldo & 1))
-        return hipErrorInvalidConfiguration;
-    const size_t lds = 2 * static_cast<size_t>(kNP) * (a.ldj + a.ldo) * sizeof(double);
+    if (a.threads != kThreads5 || 2 * a.jn > kConsumers || a.ldj > 0xffff) return hipErrorInvalidConfiguration;
+    const size_t lds = 2 * static_cast<size_t>(kNP) * a.ldj * sizeof(double);
     const bool copy = a.copy_halo != 0;
     auto kern = lay == 0 ? (copy ? sweep5_kernel<0, true> : sweep5_kernel<0, false>)
                          : (copy ? sweep5_kernel<1, true> : sweep5_kernel<1, false>);
