@@ -8,7 +8,9 @@ sweep kernel.  Checked here, 1e-13 relative (rel_err) throughout:
     way into LDS) and reshaped input, core-only and fused halo copy (halo
     bitwise); core-only launches rewrite the halo cells with the values they
     hold, so arbitrary halo contents must come back bit for bit;
-  * rows longer than one tile (ns = 312 along each axis in turn: two chunks);
+  * rows longer than one tile (ns = 312 along each axis in turn: two chunks),
+    extents that leave partial tiles (z sweeps tile 8 x-values x 2 adjacent
+    y-rows, so that the two lane groups write adjacent output rows);
   * against the general kernel (variant bit 8) on the same inputs;
   * fields scaled by 1e22 and 1e-30 (eps 1e-100): the shared denominator
     (sum_L)(sum_R) over- or underflows, so lanes take the per-stencil IEEE
@@ -126,6 +128,13 @@ def test_core_only_keeps_arbitrary_halo():
         v_rs = oracle.reshape(og, d, f)
         eL, eR = oracle.weno(og, 5, d, v_rs, cs, pL, pR)
         assert rel_err(L, eL) <= 1e-13 and rel_err(R, eR) <= 1e-13, d
+
+
+def test_odd_extents_and_partial_tiles():
+    """nx = 153 and ny = 143: partial 16-pencil tiles (x, y sweeps) and
+    partial 8 x 2 tiles of the z sweep (odd y count)."""
+    g, og, f = _case(140, 130, 127, 1, seed=512)
+    _check_box(g, og, f, seed=21)
 
 
 @pytest.mark.parametrize("axis", [0, 1, 2])

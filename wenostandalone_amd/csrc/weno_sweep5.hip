@@ -136,23 +136,41 @@ __device__ __noinline__ void cell5_slow(double v0, double v1, double v2, double 
     oR = v2 + NR / SR;
 }
 
-// Tile t of the launch -> first pencil and variable.
-__device__ __forceinline__ void tile_origin(const TileArgs& a, int t, int& iv, int& P0)
+// Tile t of the launch -> variable and tile index within it.
+__device__ __forceinline__ void tile_origin(const TileArgs& a, int t, int& iv, int& tv)
 {
     iv = t / a.tiles_per_var;
-    P0 = (t - iv * a.tiles_per_var) * kNP;
+    tv = t - iv * a.tiles_per_var;
 }
 
-// Input element offset of pencil P (within variable iv) at J = lo.
-__device__ __forceinline__ long long in_base(const TileArgs& a, int iv, int P, int lo)
+// Pencil q of tile tv -> transverse coordinates (a, b).  Pencils past the
+// end repeat the last valid one (same data, same results, same addresses).
+// Default tiles are 16 consecutive pencils in (a + na*b) order; paired tiles
+// (z sweeps) are 8 a-values x 2 adjacent b-values with q = 2*(a offset) + (b
+// offset), so pencils 2i and 2i+1 are adjacent output rows.
+template <bool PAIR>
+__device__ __forceinline__ void tile_pencil(const TileArgs& a, int tv, int q, int& pa, int& pb)
 {
-    const int pb = P / a.na, pa = P - pb * a.na;
+    if constexpr (PAIR) {
+        const int ntx = (a.na + 7) >> 3;
+        const int ty = tv / ntx, tx = tv - ty * ntx;
+        pa = min(8 * tx + (q >> 1), a.na - 1);
+        pb = min(2 * ty + (q & 1), a.nb - 1);
+    } else {
+        const int P = min(tv * kNP + q, a.na * a.nb - 1);
+        pb = P / a.na;
+        pa = P - pb * a.na;
+    }
+}
+
+// Input element offset of pencil (pa, pb) of variable iv at J = lo.
+__device__ __forceinline__ long long in_base(const TileArgs& a, int iv, int pa, int pb, int lo)
+{
     return a.src_off + pa * a.s_a + pb * a.s_b + iv * a.s_i + static_cast<long long>(lo) * a.s_j;
 }
 
-__device__ __forceinline__ long long out_base(const TileArgs& a, int iv, int P)
+__device__ __forceinline__ long long out_base(const TileArgs& a, int iv, int pa, int pb)
 {
-    const int pb = P / a.na, pa = P - pb * a.na;
     const int k = a.a_is_k ? pa : pb, l = a.a_is_k ? pb : pa;
     return k * a.o_k + l * a.o_l + iv * a.o_i;
 }
@@ -166,7 +184,7 @@ __device__ __forceinline__ long long out_base(const TileArgs& a, int iv, int P)
 //        of a tile are consecutive along x, loaded as 128-byte row segments
 //        (one pencil per lane, 4 rows per wave instruction) and transposed
 //        into LDS.
-template <int LAY, bool COPY>
+template <int LAY, bool COPY, bool PAIR>
 __global__ void __attribute__((amdgpu_flat_work_group_size(64, kThreads5), amdgpu_waves_per_eu(4)))
 sweep5_kernel(const TileArgs a)
 {
@@ -195,7 +213,6 @@ sweep5_kernel(const TileArgs a)
     const int t_first = xcd * per + min(xcd, rem) + rank;
     const int t_end = (xcd + 1) * per + min(xcd + 1, rem);
     if (t_first >= t_end) return;
-    const int npen = a.na * a.nb;
 
     const int jo0 = ch * a.chunk;
     const int jo1 = min(a.out_len, jo0 + a.chunk);
@@ -224,13 +241,22 @@ sweep5_kernel(const TileArgs a)
         const char* const src = static_cast<const char*>(a.src);
         f64x2 r2[LAY == 0 ? 2 * 14 : 1];
         double r1[LAY == 1 ? kMaxLoads : 1];
-        // halo cells, two 16-byte pieces per lane: idx = l + 64u -> pencil
-        // idx/8, side (idx/4)&1, piece idx&3 = outputs {0,1}, {2,3},
-        // {ns-4,ns-3}, {ns-2,ns-1}
+        // halo cells of vL and vR, one 16-byte piece each
         f64x2 rh[2] = {{0.0, 0.0}, {0.0, 0.0}};
         auto tile_in = [&](int t) {
-            int iv, P0;
-            tile_origin(a, t, iv, P0);
+            int iv, tv;
+            tile_origin(a, t, iv, tv);
+            // halo cells: lane -> pencil l/4, piece l&3 = outputs {0,1}, {2,3},
+            // {ns-4,ns-3}, {ns-2,ns-1}; byte offset into vL and vR
+            unsigned hoff = 0;
+            if constexpr (!COPY) {
+                const int hq = l >> 2, hpc = l & 3;
+                const int hj = hpc < 2 ? 2 * hpc : a.out_len - 4 + 2 * (hpc - 2);
+                const bool hin = hj >= jo0 && hj < jo1;
+                int pa, pb;
+                tile_pencil<PAIR>(a, tv, hq, pa, pb);
+                hoff = static_cast<unsigned>((out_base(a, iv, pa, pb) + (hin ? hj : jo0)) * sizeof(double));
+            }
             if constexpr (LAY == 0) {
                 // 8 lanes per row (128-byte segments), rows l/8 and l/8 + 8,
                 // 14 pieces each (224 elements >= nrow): buffer loads with the
@@ -241,9 +267,10 @@ sweep5_kernel(const TileArgs a)
                     const_cast<char*>(src), 0, static_cast<int>(a.src_bytes), 0x00020000);
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
-                    const int P = min(P0 + (l >> 3) + 8 * h, npen - 1);
+                    int pa, pb;
+                    tile_pencil<PAIR>(a, tv, (l >> 3) + 8 * h, pa, pb);
                     const unsigned voff =
-                        static_cast<unsigned>(in_base(a, iv, P, Lb) * sizeof(double)) + 16u * (l & 7);
+                        static_cast<unsigned>(in_base(a, iv, pa, pb, Lb) * sizeof(double)) + 16u * (l & 7);
 #pragma unroll
                     for (int m = 0; m < 14; ++m)
                         r2[14 * h + m] = __builtin_bit_cast(
@@ -253,9 +280,12 @@ sweep5_kernel(const TileArgs a)
                 const int q = l & (kNP - 1), J0 = l / kNP;
                 const long long step = kRPI * a.s_j * static_cast<long long>(sizeof(double));
                 const int nm = nrow / kRPI;
-                const long long b0 = in_base(a, iv, P0, Lb);
+                int pa0, pb0, pa, pb;
+                tile_pencil<PAIR>(a, tv, 0, pa0, pb0);
+                tile_pencil<PAIR>(a, tv, q, pa, pb);
+                const long long b0 = in_base(a, iv, pa0, pb0, Lb);
                 const unsigned lane_off = static_cast<unsigned>(
-                    (in_base(a, iv, min(P0 + q, npen - 1), Lb) - b0 + J0 * a.s_j) * sizeof(double));
+                    (in_base(a, iv, pa, pb, Lb) - b0 + J0 * a.s_j) * sizeof(double));
                 // uniform row-group address on the scalar unit; groups past
                 // the last one re-read it.  Per-tile copies of the counts keep
                 // the per-row selects inside the tile loop.
@@ -270,24 +300,19 @@ sweep5_kernel(const TileArgs a)
                     asm volatile("" : "+s"(bm));
                 }
             }
-            // then the halo cells of the tile's output rows (issued last, so
-            // address arithmetic of the next tile never waits on them)
+            // then the halo cells of the tile's output rows, vL and vR at the
+            // same 32-bit offset (issued last, so the next tile's address
+            // arithmetic never waits on them)
             if constexpr (!COPY) {
-#pragma unroll
-                for (int u = 0; u < 2; ++u) {
-                    const int idx = l + 64 * u;
-                    const int hq = idx >> 3, hside = (idx >> 2) & 1, hpc = idx & 3;
-                    const int hj = hpc < 2 ? 2 * hpc : a.out_len - 4 + 2 * (hpc - 2);
-                    const bool hin = hj >= jo0 && hj < jo1;
-                    const unsigned long long base = reinterpret_cast<unsigned long long>(hside ? a.outR : a.outL);
-                    const long long ob = out_base(a, iv, min(P0 + hq, npen - 1)) + (hin ? hj : jo0);
-                    rh[u] = *reinterpret_cast<const GlobalF64x2*>(base + ob * sizeof(double));
-                }
+                const unsigned long long bl = reinterpret_cast<unsigned long long>(a.outL);
+                const unsigned long long br = reinterpret_cast<unsigned long long>(a.outR);
+                rh[0] = *reinterpret_cast<const GlobalF64x2*>(bl + hoff);
+                rh[1] = *reinterpret_cast<const GlobalF64x2*>(br + hoff);
             }
         };
         auto commit = [&](int t, double* buf, int slot) {
-            int iv, P0;
-            tile_origin(a, t, iv, P0);
+            int iv, tv;
+            tile_origin(a, t, iv, tv);
             if constexpr (LAY == 0) {
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
@@ -306,20 +331,19 @@ sweep5_kernel(const TileArgs a)
                 }
             }
             if constexpr (!COPY) {
+                const int hq = l >> 2, hpc = l & 3;
 #pragma unroll
-                for (int u = 0; u < 2; ++u) {
-                    const int idx = l + 64 * u;
-                    const int hq = idx >> 3, hside = (idx >> 2) & 1, hpc = idx & 3;
+                for (int side = 0; side < 2; ++side) {
                     // positions 0-2: outputs 0-2, 3-5: ns-3..ns-1; pieces {2,3}
                     // and {ns-4,ns-3} carry one reconstructed output each
-                    if (hpc != 2) s_halo[slot][hq][hpc < 2 ? 2 * hpc : 2 * hpc - 2][hside] = rh[u].x;
-                    if (hpc != 1) s_halo[slot][hq][hpc < 2 ? 2 * hpc + 1 : 2 * hpc - 1][hside] = rh[u].y;
+                    if (hpc != 2) s_halo[slot][hq][hpc < 2 ? 2 * hpc : 2 * hpc - 2][side] = rh[side].x;
+                    if (hpc != 1) s_halo[slot][hq][hpc < 2 ? 2 * hpc + 1 : 2 * hpc - 1][side] = rh[side].y;
                 }
             }
-            // pencils past the end repeat the last one (same data, same
-            // results, same addresses)
             if (l < kNP) {
-                const unsigned long long ob = out_base(a, iv, min(P0 + l, npen - 1)) * sizeof(double);
+                int pa, pb;
+                tile_pencil<PAIR>(a, tv, l, pa, pb);
+                const unsigned long long ob = out_base(a, iv, pa, pb) * sizeof(double);
                 s_ptr[slot][l][0] = reinterpret_cast<unsigned long long>(a.outL) + ob;
                 s_ptr[slot][l][1] = reinterpret_cast<unsigned long long>(a.outR) + ob;
             }
@@ -414,6 +438,7 @@ int plan_sweep5(TileArgs& a, Launch& l, const void* src)
     const int ns = a.out_len;
     if (a.j_beg != 3 || a.j_end != ns - 4 || (ns & 1) || ((a.o_k | a.o_l | a.o_i) & 1)) return -1;
     if ((reinterpret_cast<uintptr_t>(a.outL) | reinterpret_cast<uintptr_t>(a.outR)) & 15) return -1;
+    if (static_cast<unsigned long long>(a.o_i) * a.nv * sizeof(double) > 0xffffffffull) return -1;
     int lay;
     if (a.s_j == 1) {
         if ((reinterpret_cast<uintptr_t>(src) & 15) || ((a.src_off | a.s_a | a.s_b | a.s_i) & 1)) return -1;
@@ -462,8 +487,10 @@ int plan_sweep5(TileArgs& a, Launch& l, const void* src)
         a.npg = kNPG;
         a.ldj = lay == 0 ? kRows0 : (rows | 1);
         a.threads = kThreads5;
+        // z sweeps: output rows are adjacent along b, so tiles pair b-rows
+        a.pair_b = (lay == 1 && !a.a_is_k) ? 1 : 0;
         const long long pen = static_cast<long long>(a.na) * a.nb;
-        a.tiles_per_var = static_cast<int>((pen + kNP - 1) / kNP);
+        a.tiles_per_var = a.pair_b ? ((a.na + 7) / 8) * ((a.nb + 1) / 2) : static_cast<int>((pen + kNP - 1) / kNP);
         return lay;
     }
     return -1;
@@ -474,8 +501,9 @@ hipError_t launch_sweep5(TileArgs a, int lay, hipStream_t st)
     if (a.threads != kThreads5 || 2 * a.jn > kConsumers || a.ldj > 0xffff) return hipErrorInvalidConfiguration;
     const size_t lds = 2 * static_cast<size_t>(kNP) * a.ldj * sizeof(double);
     const bool copy = a.copy_halo != 0;
-    auto kern = lay == 0 ? (copy ? sweep5_kernel<0, true> : sweep5_kernel<0, false>)
-                         : (copy ? sweep5_kernel<1, true> : sweep5_kernel<1, false>);
+    auto kern = lay == 0 ? (copy ? sweep5_kernel<0, true, false> : sweep5_kernel<0, false, false>)
+                : a.pair_b ? (copy ? sweep5_kernel<1, true, true> : sweep5_kernel<1, false, true>)
+                           : (copy ? sweep5_kernel<1, true, false> : sweep5_kernel<1, false, false>);
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
