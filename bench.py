@@ -36,8 +36,9 @@ HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
 # `bench.py --record-traffic FETCH_CSV WRITE_CSV`, used only while the kernel
 # sources hash the same as when it was measured
 TRAFFIC_RECORD = os.path.join(ROOT, "profiles", "sweep_weno5_fp64_200cube_traffic.json")
-KERNEL_SOURCES = ["weno_kernels.hip", "weno_device.h", "weno_internal.h", "weno_capi.hip", "weno_tune.hip"]
-SWEEP_KERNEL = "sweep_kernel<2, 5, 1, false, double, double"
+KERNEL_SOURCES = ["weno_sweep5.hip", "weno_kernels.hip", "weno_device.h", "weno_internal.h", "weno_capi.hip",
+                  "weno_tune.hip"]
+SWEEP_KERNEL = "sweep5_kernel<"
 
 
 def kernel_source_hash() -> str:
@@ -452,7 +453,7 @@ def main():
         "wall_s_timed_region": wall,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "sweep_kernel<OP_WENO,5,JS,noMP,double,double>",
+                     "kernel": "sweep5_kernel (WENO5-JS FP64, producer/consumer waves)",
                      "bytes_per_launch": bytes_launch, "avg_launch_ms": launch_ms},
     }
     out["roofline"]["traffic"] = recorded_traffic() if m == 199 and args.flags == 0 else None
