@@ -179,7 +179,9 @@ def suite(path):
     """Secondary measurements (written to `path` as JSON):
       precision  - WENO5/7 at 200^3x8 on the recipe-2 field (1e3 jumps and
                    exactly constant boxes): FP64 vs FP32 state vs BF16 state,
-                   time per x+y+z and per-output max / RMS relative error
+                   WENO5 also through the general kernel (FP32 stencil
+                   arithmetic), time per x+y+z and per-output max / RMS
+                   relative error
                    against the FP64 GPU path (itself held to 1e-13 of the
                    oracle by the test suite);
       batched    - 64 blocks, sides U{16..128} per axis, v in {5,8,10},
@@ -218,7 +220,11 @@ def suite(path):
     for order in (5, 7):
         ref = None
         row = {}
-        for dt, name in ((wm.F64, "fp64"), (wm.F32, "fp32"), (wm.BF16, "bf16")):
+        forms = [(wm.F64, "fp64", -1), (wm.F32, "fp32", -1), (wm.BF16, "bf16", -1)]
+        if order == 5:
+            # the general kernel's form: stencil arithmetic in FP32
+            forms += [(wm.F32, "fp32_general_kernel", 8 | 2), (wm.BF16, "bf16_general_kernel", 8 | 2)]
+        for dt, name, variant in forms:
             fc = faces[wm.F64 if dt == wm.F64 else wm.F32]
 
             def sweep():
@@ -226,7 +232,13 @@ def suite(path):
                     wm.check(wm.lib.weno_reconstruct(C.byref(g), order, d, wm.SRC_FIELD, dt, srcs[dt].vp,
                                                      C.byref(coefs[d][1]), fc[d][0].vp, fc[d][1].vp,
                                                      wm.FUSE_COPY))
-            ms = _time(h, stream, sweep)
+            for d in range(3):
+                wm.check(wm.lib.weno_set_tile(order, dt, d, 0, 0, 0, variant))
+            try:
+                ms = _time(h, stream, sweep)
+            finally:
+                for d in range(3):
+                    wm.check(wm.lib.weno_set_tile(order, dt, d, 0, 0, 0, -1))
             ent = {"ms_xyz": ms}
             if dt == wm.F64:
                 ref = [(L.download(), R.download()) for L, R in fc]
@@ -241,8 +253,9 @@ def suite(path):
                         del got, rel
                 ent["rel_err_vs_fp64"] = errs
             row[name] = ent
-        row["speedup_fp32"] = row["fp64"]["ms_xyz"] / row["fp32"]["ms_xyz"]
-        row["speedup_bf16"] = row["fp64"]["ms_xyz"] / row["bf16"]["ms_xyz"]
+        for name in list(row):
+            if name != "fp64":
+                row[f"speedup_{name}"] = row["fp64"]["ms_xyz"] / row[name]["ms_xyz"]
         prec[f"weno{order}"] = row
     report["precision"] = prec
     for dt in faces:

@@ -15,7 +15,11 @@ sweep kernel.  Checked here, 1e-13 relative (rel_err) throughout:
   * fields scaled by 1e22 and 1e-30 (eps 1e-100): the shared denominator
     (sum_L)(sum_R) over- or underflows, so lanes take the per-stencil IEEE
     path; isolated 1e22 cells mix both paths inside one wavefront;
-  * NaN input cells: NaN in exactly the outputs whose stencil touches them.
+  * NaN input cells: NaN in exactly the outputs whose stencil touches them;
+  * FP32 state and BF16 state (FP32 faces): the kernel widens the state to
+    FP64 as it fills LDS and runs the FP64 arithmetic, so its faces are the
+    FP64 oracle's faces for the widened (rounded) state, rounded once to FP32:
+    within one FP32 ulp (1.2e-7 relative, rel_err); halo cells bitwise.
 """
 import ctypes as C
 
@@ -44,17 +48,19 @@ def _coef(cs):
     return devs, wm.Coeffs(*[x.ptr for x in devs])
 
 
-def _run(g, d, src, kind, coef, flags, pre=None):
+def _run(g, d, src, kind, coef, flags, pre=None, dtype=None):
+    dtype = wm.F64 if dtype is None else dtype
     ns, nk, nl = fields.dir_extents(g.m_cells, g.n_cells, g.p_cells, g.buff_size, d)
-    vL = DeviceArray((ns, nk, nl, g.v_size))
-    vR = DeviceArray((ns, nk, nl, g.v_size))
+    ft = np.float64 if dtype == wm.F64 else np.float32
+    vL = DeviceArray((ns, nk, nl, g.v_size), ft)
+    vR = DeviceArray((ns, nk, nl, g.v_size), ft)
     if pre is None:
         vL.fill(7.25)
         vR.fill(-3.5)
     else:
         vL.upload(pre[0])
         vR.upload(pre[1])
-    wm.check(wm.lib.weno_reconstruct(C.byref(g), 5, d, kind, wm.F64, src.vp, C.byref(coef), vL.vp, vR.vp, flags))
+    wm.check(wm.lib.weno_reconstruct(C.byref(g), 5, d, kind, dtype, src.vp, C.byref(coef), vL.vp, vR.vp, flags))
     return vL.download(), vR.download()
 
 
@@ -180,3 +186,77 @@ def test_nan_cells_poison_their_stencils_only():
     cells = [(20, 30, 33, 0), (140, 10, 12, 0), (5, 60, 130, 0)]
     g, og, f = _case(*BOX, 1, seed=560, nan_cells=cells)
     _check_box(g, og, f, seed=61)
+
+
+def _bf16_bits(x):
+    """Round to BF16 (nearest even) and return the 16-bit patterns."""
+    u = np.asarray(x, dtype=np.float32).view(np.uint32)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def _bf16_widen(bits):
+    return (np.asarray(bits).astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("state", ["f32", "bf16"])
+def test_reduced_storage_all_sweeps(state):
+    """FP32 / BF16 state on the same boxes: field and reshaped input,
+    core-only (arbitrary FP32 halo kept bitwise) and fused copy."""
+    g, og, f = _case(*BOX, 2, seed=570)
+    if state == "f32":
+        st = np.asfortranarray(f.astype(np.float32))
+        fw = st.astype(np.float64)
+        dt = wm.F32
+    else:
+        st = np.asfortranarray(_bf16_bits(f))
+        fw = _bf16_widen(st)
+        dt = wm.BF16
+    vf = DeviceArray.from_host(st)
+    rng = np.random.default_rng(17)
+    m, n, p = BOX
+    for d in range(3):
+        ns, nk, nl = fields.dir_extents(m, n, p, 6, d)
+        cs = fields.harness_coefficients(ns - 6, fields.DIR_OFFSETS[d])
+        devs, coef = _coef(cs)
+        v_rs = oracle.reshape(og, d, fw)
+        rs_host = v_rs.astype(np.float32) if state == "f32" else _bf16_bits(v_rs)
+        rs = DeviceArray.from_host(rs_host)
+        cL, cR = oracle.order1_copy(og, d, fw)
+        pL = np.asfortranarray(rng.standard_normal((ns, nk, nl, 2)).astype(np.float32))
+        pR = np.asfortranarray(rng.standard_normal((ns, nk, nl, 2)).astype(np.float32))
+        for src, kind in [(vf, wm.SRC_FIELD), (rs, wm.SRC_RESHAPED)]:
+            L, R = _run(g, d, src, kind, coef, 0, pre=(pL, pR), dtype=dt)
+            halo = np.r_[0:3, ns - 3:ns]
+            assert np.array_equal(L[halo], pL[halo]) and np.array_equal(R[halo], pR[halo]), (d, kind)
+            eL, eR = oracle.weno(og, 5, d, v_rs, cs, pL.astype(np.float64), pR.astype(np.float64))
+            assert rel_err(L, eL) <= 1.2e-7 and rel_err(R, eR) <= 1.2e-7, (d, kind, "core")
+            L, R = _run(g, d, src, kind, coef, wm.FUSE_COPY, dtype=dt)
+            eL, eR = oracle.weno(og, 5, d, v_rs, cs, cL, cR)
+            assert rel_err(L, eL) <= 1.2e-7 and rel_err(R, eR) <= 1.2e-7, (d, kind, "copy")
+            assert np.array_equal(L[halo], cL[halo].astype(np.float32))
+            assert np.array_equal(R[halo], cR[halo].astype(np.float32))
+
+
+def test_fp32_storage_matches_general_kernel_and_nan():
+    """FP32 state through both kernels (variant bit 8 keeps the general one,
+    whose stencil arithmetic is FP32): agreement to the FP32 form's 1e-6,
+    NaN cells poisoning the same outputs."""
+    cells = [(20, 30, 33, 0), (140, 10, 12, 0)]
+    g, og, f = _case(*BOX, 1, seed=580, nan_cells=cells)
+    st = np.asfortranarray(f.astype(np.float32))
+    vf = DeviceArray.from_host(st)
+    for d in range(3):
+        ns = fields.dir_extents(*BOX, 6, d)[0]
+        cs = fields.harness_coefficients(ns - 6, fields.DIR_OFFSETS[d])
+        devs, coef = _coef(cs)
+        fast = _run(g, d, vf, wm.SRC_FIELD, coef, wm.FUSE_COPY, dtype=wm.F32)
+        wm.check(wm.lib.weno_set_tile(5, wm.F32, d, 0, 0, 0, 8 | 2))
+        try:
+            gen = _run(g, d, vf, wm.SRC_FIELD, coef, wm.FUSE_COPY, dtype=wm.F32)
+        finally:
+            wm.check(wm.lib.weno_set_tile(5, wm.F32, d, 0, 0, 0, -1))
+        assert rel_err(fast[0], gen[0]) <= 1e-6 and rel_err(fast[1], gen[1]) <= 1e-6, d
+        v_rs = oracle.reshape(og, d, st.astype(np.float64))
+        cL, cR = oracle.order1_copy(og, d, st.astype(np.float64))
+        eL, eR = oracle.weno(og, 5, d, v_rs, cs, cL, cR)
+        assert rel_err(fast[0], eL) <= 1.2e-7 and rel_err(fast[1], eR) <= 1.2e-7, d

@@ -1,6 +1,9 @@
 // HIP kernels for gfx950: the sweep kernel behind reshape, order-1 copy,
 // WENO3/5/7 (JS and TENO), the fused or standalone monotonicity-preserving
-// limiter, in FP64, FP32 and BF16-state storage.
+// limiter, in FP64, FP32 and BF16-state storage.  WENO5-JS launches that
+// write both sides over the full reconstructed range (rows of at least 96
+// outputs, aligned arrays) run on the producer/consumer kernel of
+// weno_sweep5.hip instead (launch_tile hands them over).
 //
 // Design (one kernel template, one workgroup = one tile):
 //   * A tile is NP pencils x one j-chunk of one variable.  The pencils of a
@@ -373,7 +376,7 @@ int max_threads(int op, int order);
 hipError_t launch_tile(const TileArgs& a, const Launch& l, hipStream_t st)
 {
     if (a.tiles_per_var <= 0 || a.nv <= 0) return hipSuccess;
-    if (l.fast5 > 0) return launch_sweep5(a, l.fast5 - 1, st);
+    if (l.fast5 > 0) return launch_sweep5(a, l.fast5 - 1, l.dtype, st);
     if (a.np > kMaxNP || a.slots > kMaxSlots || a.threads > max_threads(l.op, l.order) || a.threads < 64 ||
         a.ldj > 0xffff)
         return hipErrorInvalidConfiguration;

@@ -1,5 +1,11 @@
-// The FP64 WENO5-JS sweep kernel for gfx950 (the reference's hot path:
-// s_weno5_*_kernel, inputs/weno_standalone.f90:477-1417, in its JS form).
+// The WENO5-JS sweep kernel for gfx950 (the reference's hot path:
+// s_weno5_*_kernel, inputs/weno_standalone.f90:477-1417, in its JS form),
+// for FP64 state and faces, FP32 state and faces, and BF16 state with FP32
+// faces.  The tile is converted to FP64 on its way into LDS, so the
+// arithmetic below is FP64 for every storage type: on gfx950 an FP32 FMA
+// (packed FP32 is not used) takes the same issue slot as an FP64 FMA, and
+// FP32 polynomial terms would need the weights normalised before the sum
+// (six more multiplies and six conversions per face pair).
 //
 // One workgroup = 7 consumer waves + 1 producer wave, persistent over a
 // contiguous-per-XCD range of tiles.  A tile is 16 pencils x one j-chunk of
@@ -54,12 +60,88 @@ constexpr int kThreads5 = kConsumers + 64;
 constexpr int kMaxOut = 224;    // outputs per chunk row
 constexpr int kRPI = 64 / kNP;  // strided rows: rows per wave instruction (4)
 constexpr int kMaxLoads = 54;   // strided rows: producer registers, 4 x 54 = 216 rows
-constexpr int kRows0 = 224;     // contiguous rows: 14 16-byte pieces x 8 lanes per row
+constexpr int kRows0 = 224;     // contiguous rows: elements loaded per row
 
-typedef __attribute__((address_space(1))) const double GlobalF64;
+template <typename T>
+using GlobalIn = __attribute__((address_space(1))) const T;
+template <typename T>
+using GlobalOut = __attribute__((address_space(1))) T;
 typedef double f64x2 __attribute__((ext_vector_type(2)));
-typedef __attribute__((address_space(1))) const f64x2 GlobalF64x2;
-typedef __attribute__((address_space(1))) double GlobalOut;
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+
+// State storage types.  The tile is held in LDS as FP64 whatever the state
+// type (the producer converts as it commits), so the consumers run the same
+// FP64 arithmetic for every storage type; only the face type changes.
+// Contiguous rows are loaded in pieces of two elements (rows of v_vf start
+// at even elements only), LPR lanes per row:
+//   face : vL/vR element type
+//   piece: one load of two state elements (PB bytes)
+//   raw  : element as loaded from a strided pencil
+//   hp   : two adjacent faces (halo cells are loaded in such pieces)
+template <typename TS>
+struct Store5;
+template <>
+struct Store5<double> {
+    using face = double;
+    using piece = u32x4;
+    using raw = double;
+    using hp = f64x2;
+    static constexpr int PB = 16, LPR = 8;
+};
+template <>
+struct Store5<float> {
+    using face = float;
+    using piece = u32x2;
+    using raw = float;
+    using hp = f32x2;
+    static constexpr int PB = 8, LPR = 16;
+};
+template <>
+struct Store5<bf16_t> {
+    using face = float;
+    using piece = unsigned;
+    using raw = unsigned short;
+    using hp = f32x2;
+    static constexpr int PB = 4, LPR = 16;
+};
+
+__device__ __forceinline__ double widen(double x) { return x; }
+__device__ __forceinline__ double widen(float x) { return static_cast<double>(x); }
+__device__ __forceinline__ double widen(unsigned short x)
+{
+    return static_cast<double>(__uint_as_float(static_cast<unsigned>(x) << 16));
+}
+
+// One load piece of a contiguous row -> two consecutive LDS doubles.
+__device__ __forceinline__ void put_piece(double* dst, u32x4 p, double)
+{
+    *reinterpret_cast<f64x2*>(dst) = __builtin_bit_cast(f64x2, p);
+}
+__device__ __forceinline__ void put_piece(double* dst, u32x2 p, float)
+{
+    *reinterpret_cast<f64x2*>(dst) =
+        f64x2{static_cast<double>(__uint_as_float(p.x)), static_cast<double>(__uint_as_float(p.y))};
+}
+__device__ __forceinline__ void put_piece(double* dst, unsigned p, bf16_t)
+{
+    *reinterpret_cast<f64x2*>(dst) = f64x2{static_cast<double>(__uint_as_float(p << 16)),
+                                           static_cast<double>(__uint_as_float(p & 0xffff0000u))};
+}
+
+__device__ __forceinline__ u32x4 load_piece(__amdgpu_buffer_rsrc_t r, unsigned off, u32x4)
+{
+    return __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+__device__ __forceinline__ u32x2 load_piece(__amdgpu_buffer_rsrc_t r, unsigned off, u32x2)
+{
+    return __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ unsigned load_piece(__amdgpu_buffer_rsrc_t r, unsigned off, unsigned)
+{
+    return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+}
 
 struct Coef5 {
     double eL[3], fL[3], eR[3], fR[3];   // d_r * poly(r, 0..1)
@@ -184,15 +266,24 @@ __device__ __forceinline__ long long out_base(const TileArgs& a, int iv, int pa,
 //        of a tile are consecutive along x, loaded as 128-byte row segments
 //        (one pencil per lane, 4 rows per wave instruction) and transposed
 //        into LDS.
-template <int LAY, bool COPY, bool PAIR>
+template <typename TS, int LAY, bool COPY, bool PAIR>
 __global__ void __attribute__((amdgpu_flat_work_group_size(64, kThreads5), amdgpu_waves_per_eu(4)))
 sweep5_kernel(const TileArgs a)
 {
+    using TO = typename Store5<TS>::face;
+    using Piece = typename Store5<TS>::piece;
+    using Raw = typename Store5<TS>::raw;
+    using HP = typename Store5<TS>::hp;
+    constexpr int PB = Store5<TS>::PB, LPR = Store5<TS>::LPR;
+    constexpr int RPW = 64 / LPR;             // contiguous rows per wave instruction
+    constexpr int NH = kNP / RPW;             // instructions to cover the tile's rows
+    constexpr int NL0 = kRows0 / (2 * LPR);   // pieces per lane and row
+
     extern __shared__ __align__(16) double lds[];
     // per tile and pencil: byte addresses of the pencil's vL and vR rows
     __shared__ unsigned long long s_ptr[2][kNP][2];
     // per tile, pencil, halo position and side: the values vL, vR hold there
-    __shared__ double s_halo[2][kNP][6][2];
+    __shared__ TO s_halo[2][kNP][6][2];
 
     const int tid = threadIdx.x;
     const int nwave = blockDim.x >> 6;
@@ -239,10 +330,10 @@ sweep5_kernel(const TileArgs a)
     if (producer) {
         const int l = tid & 63;
         const char* const src = static_cast<const char*>(a.src);
-        f64x2 r2[LAY == 0 ? 2 * 14 : 1];
-        double r1[LAY == 1 ? kMaxLoads : 1];
-        // halo cells of vL and vR, one 16-byte piece each
-        f64x2 rh[2] = {{0.0, 0.0}, {0.0, 0.0}};
+        Piece r2[LAY == 0 ? NH * NL0 : 1];
+        Raw r1[LAY == 1 ? kMaxLoads : 1];
+        // halo cells of vL and vR, one piece of two faces each
+        HP rh[2] = {{0, 0}, {0, 0}};
         auto tile_in = [&](int t) {
             int iv, tv;
             tile_origin(a, t, iv, tv);
@@ -255,47 +346,47 @@ sweep5_kernel(const TileArgs a)
                 const bool hin = hj >= jo0 && hj < jo1;
                 int pa, pb;
                 tile_pencil<PAIR>(a, tv, hq, pa, pb);
-                hoff = static_cast<unsigned>((out_base(a, iv, pa, pb) + (hin ? hj : jo0)) * sizeof(double));
+                hoff = static_cast<unsigned>((out_base(a, iv, pa, pb) + (hin ? hj : jo0)) * sizeof(TO));
             }
             if constexpr (LAY == 0) {
-                // 8 lanes per row (128-byte segments), rows l/8 and l/8 + 8,
-                // 14 pieces each (224 elements >= nrow): buffer loads with the
-                // array size as the range, so pieces past the row end read the
-                // next row or, at the very end of the array, zero; they land
-                // in the LDS row padding (ldj = 224) and are never read
+                // LPR lanes per row (LPR consecutive pieces), rows
+                // l/LPR + RPW*h, NL0 pieces each (224 elements >= nrow):
+                // buffer loads with the array size as the range, so pieces
+                // past the row end read the next row or, at the very end of
+                // the array, zero; they land in the LDS row padding (ldj =
+                // 224) and are never read
                 const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
                     const_cast<char*>(src), 0, static_cast<int>(a.src_bytes), 0x00020000);
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
+                for (int h = 0; h < NH; ++h) {
                     int pa, pb;
-                    tile_pencil<PAIR>(a, tv, (l >> 3) + 8 * h, pa, pb);
-                    const unsigned voff =
-                        static_cast<unsigned>(in_base(a, iv, pa, pb, Lb) * sizeof(double)) + 16u * (l & 7);
+                    tile_pencil<PAIR>(a, tv, l / LPR + RPW * h, pa, pb);
+                    const unsigned voff = static_cast<unsigned>(in_base(a, iv, pa, pb, Lb) * sizeof(TS)) +
+                                          unsigned(PB) * (l % LPR);
 #pragma unroll
-                    for (int m = 0; m < 14; ++m)
-                        r2[14 * h + m] = __builtin_bit_cast(
-                            f64x2, __builtin_amdgcn_raw_buffer_load_b128(rsrc, voff + 128u * m, 0, 0));
+                    for (int m = 0; m < NL0; ++m)
+                        r2[NL0 * h + m] = load_piece(rsrc, voff + unsigned(LPR * PB) * m, Piece{});
                 }
             } else {
                 const int q = l & (kNP - 1), J0 = l / kNP;
-                const long long step = kRPI * a.s_j * static_cast<long long>(sizeof(double));
+                const long long step = kRPI * a.s_j * static_cast<long long>(sizeof(TS));
                 const int nm = nrow / kRPI;
                 int pa0, pb0, pa, pb;
                 tile_pencil<PAIR>(a, tv, 0, pa0, pb0);
                 tile_pencil<PAIR>(a, tv, q, pa, pb);
                 const long long b0 = in_base(a, iv, pa0, pb0, Lb);
                 const unsigned lane_off = static_cast<unsigned>(
-                    (in_base(a, iv, pa, pb, Lb) - b0 + J0 * a.s_j) * sizeof(double));
+                    (in_base(a, iv, pa, pb, Lb) - b0 + J0 * a.s_j) * sizeof(TS));
                 // uniform row-group address on the scalar unit; groups past
                 // the last one re-read it.  Per-tile copies of the counts keep
                 // the per-row selects inside the tile loop.
-                unsigned long long bm = reinterpret_cast<unsigned long long>(src) + b0 * sizeof(double);
+                unsigned long long bm = reinterpret_cast<unsigned long long>(src) + b0 * sizeof(TS);
                 int n = nm;
                 long long st = step;
                 asm volatile("" : "+s"(n), "+s"(st));
 #pragma unroll
                 for (int m = 0; m < kMaxLoads; ++m) {
-                    r1[m] = *reinterpret_cast<const GlobalF64*>(bm + lane_off);
+                    r1[m] = *reinterpret_cast<GlobalIn<Raw>*>(bm + lane_off);
                     bm += (m + 1 < n) ? st : 0;
                     asm volatile("" : "+s"(bm));
                 }
@@ -306,8 +397,8 @@ sweep5_kernel(const TileArgs a)
             if constexpr (!COPY) {
                 const unsigned long long bl = reinterpret_cast<unsigned long long>(a.outL);
                 const unsigned long long br = reinterpret_cast<unsigned long long>(a.outR);
-                rh[0] = *reinterpret_cast<const GlobalF64x2*>(bl + hoff);
-                rh[1] = *reinterpret_cast<const GlobalF64x2*>(br + hoff);
+                rh[0] = *reinterpret_cast<GlobalIn<HP>*>(bl + hoff);
+                rh[1] = *reinterpret_cast<GlobalIn<HP>*>(br + hoff);
             }
         };
         auto commit = [&](int t, double* buf, int slot) {
@@ -315,10 +406,10 @@ sweep5_kernel(const TileArgs a)
             tile_origin(a, t, iv, tv);
             if constexpr (LAY == 0) {
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    f64x2* row = reinterpret_cast<f64x2*>(buf + ((l >> 3) + 8 * h) * ldj) + (l & 7);
+                for (int h = 0; h < NH; ++h) {
+                    double* row = buf + (l / LPR + RPW * h) * ldj + 2 * (l % LPR);
 #pragma unroll
-                    for (int m = 0; m < 14; ++m) row[8 * m] = r2[14 * h + m];
+                    for (int m = 0; m < NL0; ++m) put_piece(row + 2 * LPR * m, r2[NL0 * h + m], TS{});
                 }
             } else {
                 double* row = buf + (l & (kNP - 1)) * ldj + l / kNP;
@@ -326,7 +417,7 @@ sweep5_kernel(const TileArgs a)
                 asm volatile("" : "+s"(n));
 #pragma unroll
                 for (int m = 0; m < kMaxLoads; ++m) {
-                    *row = r1[m];
+                    *row = widen(r1[m]);
                     row += (m + 1 < n) ? kRPI : 0;
                 }
             }
@@ -343,7 +434,7 @@ sweep5_kernel(const TileArgs a)
             if (l < kNP) {
                 int pa, pb;
                 tile_pencil<PAIR>(a, tv, l, pa, pb);
-                const unsigned long long ob = out_base(a, iv, pa, pb) * sizeof(double);
+                const unsigned long long ob = out_base(a, iv, pa, pb) * sizeof(TO);
                 s_ptr[slot][l][0] = reinterpret_cast<unsigned long long>(a.outL) + ob;
                 s_ptr[slot][l][1] = reinterpret_cast<unsigned long long>(a.outR) + ob;
             }
@@ -380,7 +471,7 @@ sweep5_kernel(const TileArgs a)
         load_coef5(a.c, jc, k5);
     }
     const double eps = a.eps;
-    const unsigned long long joff = static_cast<unsigned long long>(j) * sizeof(double);
+    const unsigned long long joff = static_cast<unsigned long long>(j) * sizeof(TO);
 
     __syncthreads();
     int k = 0;
@@ -397,17 +488,18 @@ sweep5_kernel(const TileArgs a)
                 const double v0 = row[-2], v1 = row[-1], v2 = row[0], v3 = row[1], v4 = row[2];
                 double oL, oR;
                 allok &= cell5(v0, v1, v2, v3, v4, k5, eps, oL, oR);
+                TO wL = static_cast<TO>(oL), wR = static_cast<TO>(oR);
                 if (halo) {
                     if constexpr (COPY) {
-                        oL = oR = row[col - colc];
+                        wL = wR = static_cast<TO>(row[col - colc]);
                     } else {
-                        oL = s_halo[slot][q][hpos][0];
-                        oR = s_halo[slot][q][hpos][1];
+                        wL = s_halo[slot][q][hpos][0];
+                        wR = s_halo[slot][q][hpos][1];
                     }
                 }
                 const unsigned long long* pp = s_ptr[slot][q];
-                __builtin_nontemporal_store(oL, reinterpret_cast<GlobalOut*>(pp[0] + joff));
-                __builtin_nontemporal_store(oR, reinterpret_cast<GlobalOut*>(pp[1] + joff));
+                __builtin_nontemporal_store(wL, reinterpret_cast<GlobalOut<TO>*>(pp[0] + joff));
+                __builtin_nontemporal_store(wR, reinterpret_cast<GlobalOut<TO>*>(pp[1] + joff));
             }
             if (!allok && core) {
 #pragma unroll 1
@@ -417,8 +509,8 @@ sweep5_kernel(const TileArgs a)
                     double oL, oR;
                     cell5_slow(row[-2], row[-1], row[0], row[1], row[2], k5, eps, oL, oR);
                     const unsigned long long* pp = s_ptr[slot][q];
-                    __builtin_nontemporal_store(oL, reinterpret_cast<GlobalOut*>(pp[0] + joff));
-                    __builtin_nontemporal_store(oR, reinterpret_cast<GlobalOut*>(pp[1] + joff));
+                    __builtin_nontemporal_store(static_cast<TO>(oL), reinterpret_cast<GlobalOut<TO>*>(pp[0] + joff));
+                    __builtin_nontemporal_store(static_cast<TO>(oR), reinterpret_cast<GlobalOut<TO>*>(pp[1] + joff));
                 }
             }
         }
@@ -426,28 +518,37 @@ sweep5_kernel(const TileArgs a)
     }
 }
 
-// Host side: decides whether a WENO5-JS FP64 launch runs on this kernel and
-// sets its tile shape.  Returns the layout (0 or 1), or -1 when the general
-// sweep kernel must be used (other orders/modes/dtypes, the fused limiter,
-// one-sided output, rows shorter than 96 outputs, odd row lengths or
-// misaligned arrays, inputs too large for 32-bit offsets).
+// Host side: decides whether a WENO5-JS launch runs on this kernel and sets
+// its tile shape.  Returns the layout (0 or 1), or -1 when the general sweep
+// kernel must be used (other orders/modes, the fused limiter, one-sided
+// output, rows shorter than 96 outputs, odd row lengths or misaligned
+// arrays, arrays too large for 32-bit offsets).
 int plan_sweep5(TileArgs& a, Launch& l, const void* src)
 {
-    if (l.op != OP_WENO || l.order != 5 || l.mode != MODE_JS || l.mp || l.dtype != DT_F64 || a.sides != 3)
-        return -1;
+    if (l.op != OP_WENO || l.order != 5 || l.mode != MODE_JS || l.mp || a.sides != 3) return -1;
+    int in_b, out_b;
+    switch (l.dtype) {
+    case DT_F64: in_b = 8, out_b = 8; break;
+    case DT_F32: in_b = 4, out_b = 4; break;
+    case DT_BF16: in_b = 2, out_b = 4; break;
+    default: return -1;
+    }
     const int ns = a.out_len;
     if (a.j_beg != 3 || a.j_end != ns - 4 || (ns & 1) || ((a.o_k | a.o_l | a.o_i) & 1)) return -1;
-    if ((reinterpret_cast<uintptr_t>(a.outL) | reinterpret_cast<uintptr_t>(a.outR)) & 15) return -1;
-    if (static_cast<unsigned long long>(a.o_i) * a.nv * sizeof(double) > 0xffffffffull) return -1;
+    // halo cells are read in pieces of two faces
+    if ((reinterpret_cast<uintptr_t>(a.outL) | reinterpret_cast<uintptr_t>(a.outR)) & (2 * out_b - 1)) return -1;
+    if (static_cast<unsigned long long>(a.o_i) * a.nv * out_b > 0xffffffffull) return -1;
     int lay;
     if (a.s_j == 1) {
-        if ((reinterpret_cast<uintptr_t>(src) & 15) || ((a.src_off | a.s_a | a.s_b | a.s_i) & 1)) return -1;
+        // pieces of two elements
+        if ((reinterpret_cast<uintptr_t>(src) & (2 * in_b - 1)) || ((a.src_off | a.s_a | a.s_b | a.s_i) & 1))
+            return -1;
         // 32-bit buffer offsets
-        a.src_bytes = static_cast<unsigned long long>(a.s_i) * a.nv * sizeof(double);
+        a.src_bytes = static_cast<unsigned long long>(a.s_i) * a.nv * in_b;
         if (a.src_bytes > 0xffffffffull) return -1;
         lay = 0;
     } else if (a.s_a == 1) {
-        if ((a.s_b + kNP + kRPI * a.s_j) * static_cast<long long>(sizeof(double)) >= (1LL << 31)) return -1;
+        if ((a.s_b + kNP + kRPI * a.s_j) * static_cast<long long>(in_b) >= (1LL << 31)) return -1;
         lay = 1;
     } else {
         return -1;
@@ -496,14 +597,25 @@ int plan_sweep5(TileArgs& a, Launch& l, const void* src)
     return -1;
 }
 
-hipError_t launch_sweep5(TileArgs a, int lay, hipStream_t st)
+namespace {
+typedef void (*Sweep5Fn)(const TileArgs);
+template <typename TS>
+Sweep5Fn pick5(int lay, bool pair, bool copy)
+{
+    return lay == 0 ? (copy ? sweep5_kernel<TS, 0, true, false> : sweep5_kernel<TS, 0, false, false>)
+           : pair   ? (copy ? sweep5_kernel<TS, 1, true, true> : sweep5_kernel<TS, 1, false, true>)
+                    : (copy ? sweep5_kernel<TS, 1, true, false> : sweep5_kernel<TS, 1, false, false>);
+}
+}  // namespace
+
+hipError_t launch_sweep5(TileArgs a, int lay, int dtype, hipStream_t st)
 {
     if (a.threads != kThreads5 || 2 * a.jn > kConsumers || a.ldj > 0xffff) return hipErrorInvalidConfiguration;
     const size_t lds = 2 * static_cast<size_t>(kNP) * a.ldj * sizeof(double);
-    const bool copy = a.copy_halo != 0;
-    auto kern = lay == 0 ? (copy ? sweep5_kernel<0, true, false> : sweep5_kernel<0, false, false>)
-                : a.pair_b ? (copy ? sweep5_kernel<1, true, true> : sweep5_kernel<1, false, true>)
-                           : (copy ? sweep5_kernel<1, true, false> : sweep5_kernel<1, false, false>);
+    const bool copy = a.copy_halo != 0, pair = a.pair_b != 0;
+    const Sweep5Fn kern = dtype == DT_F64   ? pick5<double>(lay, pair, copy)
+                          : dtype == DT_F32 ? pick5<float>(lay, pair, copy)
+                                            : pick5<bf16_t>(lay, pair, copy);
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
