@@ -143,6 +143,16 @@ __device__ __forceinline__ unsigned load_piece(__amdgpu_buffer_rsrc_t r, unsigne
     return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
 }
 
+// Non-temporal face store at a 32-bit byte offset (cache policy bit 1: nt).
+__device__ __forceinline__ void store_face(double x, __amdgpu_buffer_rsrc_t r, unsigned off)
+{
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, off, 0, 2);
+}
+__device__ __forceinline__ void store_face(float x, __amdgpu_buffer_rsrc_t r, unsigned off)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), r, off, 0, 2);
+}
+
 struct Coef5 {
     double eL[3], fL[3], eR[3], fR[3];   // d_r * poly(r, 0..1)
     double b[9];                         // beta_coef(j, r, s), r-major
@@ -170,6 +180,9 @@ __device__ __forceinline__ void load_coef5(const Coefs& c, int jc, Coef5& k)
 __device__ __forceinline__ bool pos_normal(double x) { return __builtin_amdgcn_class(x, 1 << 8); }
 
 // Fast path; returns false when the shared denominator is out of range.
+// NEWTON steps refine the hardware reciprocal (about 2^-23 relative): two
+// for FP64 faces, one (2^-44) for FP32 faces.
+template <int NEWTON>
 __device__ __forceinline__ bool cell5(double v0, double v1, double v2, double v3, double v4, const Coef5& k,
                                       double eps, double& oL, double& oR)
 {
@@ -190,10 +203,11 @@ __device__ __forceinline__ bool cell5(double v0, double v1, double v2, double v3
     const double SR = fma(k.dR[0], u0, fma(k.dR[1], u1, k.dR[2] * u2));
     const double T = SL * SR;
     double r = __builtin_amdgcn_rcp(T);
-    double e = fma(-T, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-T, r, 1.0);
-    r = fma(r, e, r);
+#pragma unroll
+    for (int it = 0; it < NEWTON; ++it) {
+        const double e = fma(-T, r, 1.0);
+        r = fma(r, e, r);
+    }
     oL = fma(NL, SR * r, v2);
     oR = fma(NR, SL * r, v2);
     return pos_normal(T) & pos_normal(r);
@@ -280,8 +294,8 @@ sweep5_kernel(const TileArgs a)
     constexpr int NL0 = kRows0 / (2 * LPR);   // pieces per lane and row
 
     extern __shared__ __align__(16) double lds[];
-    // per tile and pencil: byte addresses of the pencil's vL and vR rows
-    __shared__ unsigned long long s_ptr[2][kNP][2];
+    // per tile and pencil: byte offset of the pencil's row in vL and vR
+    __shared__ unsigned s_off[2][kNP];
     // per tile, pencil, halo position and side: the values vL, vR hold there
     __shared__ TO s_halo[2][kNP][6][2];
 
@@ -434,9 +448,7 @@ sweep5_kernel(const TileArgs a)
             if (l < kNP) {
                 int pa, pb;
                 tile_pencil<PAIR>(a, tv, l, pa, pb);
-                const unsigned long long ob = out_base(a, iv, pa, pb) * sizeof(TO);
-                s_ptr[slot][l][0] = reinterpret_cast<unsigned long long>(a.outL) + ob;
-                s_ptr[slot][l][1] = reinterpret_cast<unsigned long long>(a.outR) + ob;
+                s_off[slot][l] = static_cast<unsigned>(out_base(a, iv, pa, pb) * sizeof(TO));
             }
         };
         tile_in(t_first);
@@ -471,7 +483,12 @@ sweep5_kernel(const TileArgs a)
         load_coef5(a.c, jc, k5);
     }
     const double eps = a.eps;
-    const unsigned long long joff = static_cast<unsigned long long>(j) * sizeof(TO);
+    const unsigned joff = static_cast<unsigned>(j * sizeof(TO));
+    // vL and vR as buffers (offsets below 4 GiB, checked by the planner)
+    const int out_bytes = static_cast<int>(static_cast<unsigned long long>(a.o_i) * a.nv * sizeof(TO));
+    const __amdgpu_buffer_rsrc_t rL = __builtin_amdgcn_make_buffer_rsrc(a.outL, 0, out_bytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rR = __builtin_amdgcn_make_buffer_rsrc(a.outR, 0, out_bytes, 0x00020000);
+    constexpr int NEWTON = sizeof(TO) == sizeof(double) ? 2 : 1;
 
     __syncthreads();
     int k = 0;
@@ -487,7 +504,7 @@ sweep5_kernel(const TileArgs a)
                 const double* row = cur + 2 * qq * ldj + colc;
                 const double v0 = row[-2], v1 = row[-1], v2 = row[0], v3 = row[1], v4 = row[2];
                 double oL, oR;
-                allok &= cell5(v0, v1, v2, v3, v4, k5, eps, oL, oR);
+                allok &= cell5<NEWTON>(v0, v1, v2, v3, v4, k5, eps, oL, oR);
                 TO wL = static_cast<TO>(oL), wR = static_cast<TO>(oR);
                 if (halo) {
                     if constexpr (COPY) {
@@ -497,9 +514,9 @@ sweep5_kernel(const TileArgs a)
                         wR = s_halo[slot][q][hpos][1];
                     }
                 }
-                const unsigned long long* pp = s_ptr[slot][q];
-                __builtin_nontemporal_store(wL, reinterpret_cast<GlobalOut<TO>*>(pp[0] + joff));
-                __builtin_nontemporal_store(wR, reinterpret_cast<GlobalOut<TO>*>(pp[1] + joff));
+                const unsigned off = s_off[slot][q] + joff;
+                store_face(wL, rL, off);
+                store_face(wR, rR, off);
             }
             if (!allok && core) {
 #pragma unroll 1
@@ -508,9 +525,9 @@ sweep5_kernel(const TileArgs a)
                     const double* row = cur + 2 * qq * ldj + colc;
                     double oL, oR;
                     cell5_slow(row[-2], row[-1], row[0], row[1], row[2], k5, eps, oL, oR);
-                    const unsigned long long* pp = s_ptr[slot][q];
-                    __builtin_nontemporal_store(static_cast<TO>(oL), reinterpret_cast<GlobalOut<TO>*>(pp[0] + joff));
-                    __builtin_nontemporal_store(static_cast<TO>(oR), reinterpret_cast<GlobalOut<TO>*>(pp[1] + joff));
+                    const unsigned off = s_off[slot][q] + joff;
+                    store_face(static_cast<TO>(oL), rL, off);
+                    store_face(static_cast<TO>(oR), rR, off);
                 }
             }
         }
