@@ -159,11 +159,18 @@ def _events(h, n):
     return out
 
 
-def _time(h, stream, fn, reps=5, warm=2):
-    """Mean milliseconds of fn() over reps launches, HIP events on stream."""
+def _time(h, stream, fn, reps=10, warm=2, warm_s=0.3):
+    """Mean milliseconds of fn() over reps launches, HIP events on stream,
+    after at least warm_s seconds of warm-up launches (the GPU clocks up
+    after host-side pauses such as the error checks between forms)."""
     import wenostandalone_amd as wm
     for _ in range(warm):
         fn()
+    wm.check(wm.lib.weno_sync())
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < warm_s:
+        fn()
+        wm.check(wm.lib.weno_sync())
     a, b = _events(h, 2)
     h.hipEventRecord(a, stream)
     for _ in range(reps):
@@ -342,8 +349,8 @@ def suite(path):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--m", type=int, default=199)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--flags", type=int, default=0, help="extra weno_reconstruct flags")
