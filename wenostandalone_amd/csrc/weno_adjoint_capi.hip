@@ -47,27 +47,45 @@ extern "C" int weno_reconstruct_vjp(const weno_grid* g, int order, int dir, int 
     const int nJ = jn + 2 * R;
     const int tmax = vjp_max_threads();
     if (jn > tmax) return fail(WENO_E_INVALID, "adjoint sweep length %d exceeds %d core points", jn, tmax);
+    // G lane groups of jn lanes.  The kernel's register use allows 8 waves
+    // per CU: workgroups of at most 4 waves run two per CU, so that one
+    // loads while the other computes; take that shape when it keeps at
+    // least 80% of the lanes busy, otherwise the best fill up to tmax.
     int G = 1, T = 64 * ((jn + 63) / 64);
     double best = double(jn) / T;
-    for (int gg = 2; gg <= 16; ++gg) {
-        const int t = 64 * ((gg * jn + 63) / 64);
-        if (t > tmax) break;
-        if (double(gg * jn) / t > best + 1e-9) {
-            best = double(gg * jn) / t;
-            G = gg;
-            T = t;
+    auto search = [&](int cap) {
+        G = 1;
+        T = 64 * ((jn + 63) / 64);
+        best = double(jn) / T;
+        for (int gg = 2; gg <= 16; ++gg) {
+            const int t = 64 * ((gg * jn + 63) / 64);
+            if (t > cap) break;
+            if (double(gg * jn) / t > best + 1e-9) {
+                best = double(gg * jn) / t;
+                G = gg;
+                T = t;
+            }
         }
-    }
+    };
+    search(256);
+    if (T > 256 || best < 0.8) search(tmax);
     const int entries = (order == 5) ? 15 : 32;
     const int ldj = nJ | 1;
-    const size_t lds_cap = 150 * 1024;
-    int npg = std::max(1, 8 / G);
+    // LDS per workgroup: input and gradient tiles, the contributions of
+    // the outputs (every pencil of the tile for WENO5, one per group for
+    // WENO7), the coefficient table.  Two workgroups per CU when the tile
+    // allows it (78 KB each), one otherwise.
     auto lds_of = [&](int np) {
-        return (2 * size_t(np) * ldj + size_t(G) * W * jn + size_t(entries) * jn) * sizeof(double);
+        return (2 * size_t(np) * ldj + size_t(order == 5 ? np : G) * W * jn + size_t(entries) * jn) *
+               sizeof(double);
     };
-    while (npg > 1 && (lds_of(G * npg) > lds_cap || (size_t(G) * npg * ldj + T - 1) / T > size_t(vjp_max_slots())))
-        --npg;
-    if (lds_of(G * npg) > lds_cap || (size_t(G) * npg * ldj + T - 1) / T > size_t(vjp_max_slots()))
+    auto fits = [&](int npg, size_t cap) {
+        return lds_of(G * npg) <= cap && (size_t(G) * npg * ldj + T - 1) / T <= size_t(vjp_max_slots()) &&
+               G * npg <= vjp_max_np();
+    };
+    int npg = std::max(1, (order == 5 ? 16 : 8) / G);
+    while (npg > 1 && !fits(npg, 78 * 1024)) --npg;
+    if (!fits(npg, 78 * 1024) && !fits(npg, 150 * 1024))
         return fail(WENO_E_INVALID, "adjoint tile does not fit LDS for sweep length %d", d.ns);
     a.groups = G;
     a.npg = npg;

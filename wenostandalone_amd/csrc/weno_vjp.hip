@@ -62,8 +62,13 @@ __device__ __forceinline__ void weno_vjp_cell(const double* v, const Cf<ORDER, d
                    B(row, 4) * Bv * Cv + B(row, 5) * Cv * Cv + eps;
         }
     }
-    // normalised weights (shared-product form, per-stencil fallback)
-    double qq[NR], P[NR], wL[NR], wR[NR];
+    // Normalised weights and 1/b_r.  Shared form: w_r = d_r P_r / S with
+    // P_r = prod_{s != r} b_s^2; WENO5 normalises both sides by one
+    // reciprocal of S_L S_R and takes 1/b_r = (prod_{s != r} b_s) / prod_s b_s
+    // (one more reciprocal); WENO7 (register-bound) divides per side and per
+    // stencil.  Lanes whose products leave the normal range use the
+    // per-stencil form 1/b_r^2 with IEEE-accurate divisions.
+    double qq[NR], P[NR], wL[NR], wR[NR], ib[NR];
 #pragma unroll
     for (int q = 0; q < NR; ++q) qq[q] = b[q] * b[q];
     excl_products<NR>(qq, P);
@@ -75,7 +80,38 @@ __device__ __forceinline__ void weno_vjp_cell(const double* v, const Cf<ORDER, d
         SL += wL[q];
         SR += wR[q];
     }
-    if (!(SL >= DBL_MIN && SL <= DBL_MAX && SR >= DBL_MIN && SR <= DBL_MAX)) {
+    double iL, iR;
+    if constexpr (ORDER == 7) {
+        if (!(SL >= DBL_MIN && SL <= DBL_MAX && SR >= DBL_MIN && SR <= DBL_MAX)) {
+            SL = SR = 0.0;
+#pragma unroll
+            for (int q = 0; q < NR; ++q) {
+                wL[q] = B.dl(q) / qq[q];
+                wR[q] = B.dr(q) / qq[q];
+                SL += wL[q];
+                SR += wR[q];
+            }
+        }
+        iL = fast_div(1.0, SL);
+        iR = fast_div(1.0, SR);
+    } else {
+    double mb[NR];
+    excl_products<NR>(b, mb);
+    const double T = SL * SR, Q = mb[0] * b[0];
+    double rT = __builtin_amdgcn_rcp(T), rQ = __builtin_amdgcn_rcp(Q);
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        rT = fma(rT, fma(-T, rT, 1.0), rT);
+        rQ = fma(rQ, fma(-Q, rQ, 1.0), rQ);
+    }
+    const bool shared = __builtin_amdgcn_class(T, 1 << 8) & __builtin_amdgcn_class(rT, 1 << 8) &
+                        __builtin_amdgcn_class(Q, 1 << 8) & __builtin_amdgcn_class(rQ, 1 << 8);
+    if (shared) {
+        iL = SR * rT;
+        iR = SL * rT;
+#pragma unroll
+        for (int q = 0; q < NR; ++q) ib[q] = mb[q] * rQ;
+    } else {
         SL = SR = 0.0;
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
@@ -83,9 +119,12 @@ __device__ __forceinline__ void weno_vjp_cell(const double* v, const Cf<ORDER, d
             wR[q] = B.dr(q) / qq[q];
             SL += wL[q];
             SR += wR[q];
+            ib[q] = 1.0 / b[q];
         }
+        iL = 1.0 / SL;
+        iR = 1.0 / SR;
     }
-    const double iL = fast_div(1.0, SL), iR = fast_div(1.0, SR);
+    }
     double vLo = 0.0, vRo = 0.0;
 #pragma unroll
     for (int q = 0; q < NR; ++q) {
@@ -98,7 +137,11 @@ __device__ __forceinline__ void weno_vjp_cell(const double* v, const Cf<ORDER, d
     for (int q = 0; q < NR; ++q) {
         const int row = (ORDER == 5) ? q : 3 - q;
         const double gpL = gl * wL[q], gpR = gr * wR[q];
-        const double gb = -2.0 * fast_div(fma(gpL, pL[q] - vLo, gpR * (pR[q] - vRo)), b[q]);
+        double gb;
+        if constexpr (ORDER == 7)
+            gb = -2.0 * fast_div(fma(gpL, pL[q] - vLo, gpR * (pR[q] - vRo)), b[q]);
+        else
+            gb = -2.0 * fma(gpL, pL[q] - vLo, gpR * (pR[q] - vRo)) * ib[q];
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const int k = (ORDER == 5) ? (1 - q - t) : (q - 1 - t);
@@ -142,8 +185,9 @@ __global__ void __launch_bounds__(kVjpThreads) vjp_kernel(const TileArgs a, doub
     const int np = a.np, ldj = a.ldj, G = a.groups, jn = a.jn;
     double* const tile = reinterpret_cast<double*>(smem_raw);     // [np][ldj] inputs
     double* const gtile = tile + np * ldj;                         // [np][ldj] gradient
-    double* const cbuf = gtile + np * ldj;                         // [G][W][jn] contributions
-    double* const beta_lds = cbuf + G * W * jn;                    // coefficient table
+    constexpr bool ALL = ORDER == 5;
+    double* const cbuf = gtile + np * ldj;                         // [np or G][W][jn] contributions
+    double* const beta_lds = cbuf + (ALL ? np : G) * W * jn;       // coefficient table
     __shared__ long long s_in[kVjpMaxNP];
     __shared__ long long s_out[kVjpMaxNP];
 
@@ -217,36 +261,47 @@ __global__ void __launch_bounds__(kVjpThreads) vjp_kernel(const TileArgs a, doub
             tile[q * ldj + J] = b >= 0 ? src[b + J * a.s_j] : 0.0;
         }
         __syncthreads();
-        for (int qq = 0; qq < a.npg; ++qq) {
-            const int q = gq * a.npg + qq;
-            double* const cb = cbuf + gq * W * jn;
-            if (core) {
-                const long long ob = s_out[q];
-                double c[W];
-                if (ob >= 0) {
-                    const double* bp = beta_lds + jj;
-                    asm volatile("" : "+v"(bp));
-                    const BetaRef<NS> Bref{bp, jn, NB, NR};
-                    weno_vjp_cell<ORDER>(tile + q * ldj + jj, f, Bref, a.eps, gL[ob + j], gR[ob + j], c);
-                } else {
+        // contributions of the outputs, one barrier, the sums per input
+        // position, one barrier: WENO5 stages every pencil of the tile at
+        // once, WENO7 (register-bound) one pencil per group at a time
+        for (int q0 = 0; q0 < a.npg; q0 += (ALL ? a.npg : 1)) {
+            const int q1 = ALL ? a.npg : q0 + 1;
+#pragma unroll 1
+            for (int qq = q0; qq < q1; ++qq) {
+                const int q = gq * a.npg + qq;
+                double* const cb = cbuf + (ALL ? q : gq) * W * jn;
+                if (core) {
+                    const long long ob = s_out[q];
+                    double c[W];
+                    if (ob >= 0) {
+                        const double* bp = beta_lds + jj;
+                        asm volatile("" : "+v"(bp));
+                        const BetaRef<NS> Bref{bp, jn, NB, NR};
+                        weno_vjp_cell<ORDER>(tile + q * ldj + jj, f, Bref, a.eps, gL[ob + j], gR[ob + j], c);
+                    } else {
 #pragma unroll
-                    for (int o = 0; o < W; ++o) c[o] = 0.0;
+                        for (int o = 0; o < W; ++o) c[o] = 0.0;
+                    }
+#pragma unroll
+                    for (int o = 0; o < W; ++o) cb[o * jn + jj] = c[o];
                 }
-#pragma unroll
-                for (int o = 0; o < W; ++o) cb[o * jn + jj] = c[o];
             }
             __syncthreads();
             if (grp) {
                 const int ncore = c1 - c0 + 1;
-                for (int m = jj; m < nJ; m += jn) {
-                    // position m receives c[o] of output jj' = m - o
-                    double sum = 0.0;
+                for (int qq = q0; qq < q1; ++qq) {
+                    const int q = gq * a.npg + qq;
+                    const double* const cb = cbuf + (ALL ? q : gq) * W * jn;
+                    for (int m = jj; m < nJ; m += jn) {
+                        // position m receives c[o] of output jj' = m - o
+                        double sum = 0.0;
 #pragma unroll
-                    for (int o = 0; o < W; ++o) {
-                        const int js = m - o;
-                        if (js >= 0 && js < ncore) sum += cb[o * jn + js];
+                        for (int o = 0; o < W; ++o) {
+                            const int js = m - o;
+                            if (js >= 0 && js < ncore) sum += cb[o * jn + js];
+                        }
+                        gtile[q * ldj + m] = sum;
                     }
-                    gtile[q * ldj + m] = sum;
                 }
             }
             __syncthreads();
@@ -272,7 +327,8 @@ __global__ void __launch_bounds__(kVjpThreads) vjp_kernel(const TileArgs a, doub
 
 hipError_t launch_vjp(TileArgs a, int order, double* grad, hipStream_t st)
 {
-    const size_t lds = (2 * static_cast<size_t>(a.np) * a.ldj + static_cast<size_t>(a.groups) * (order + 0) * a.jn +
+    const size_t lds = (2 * static_cast<size_t>(a.np) * a.ldj +
+                        static_cast<size_t>(order == 5 ? a.np : a.groups) * order * a.jn +
                         static_cast<size_t>(order == 5 ? BetaTraits<5>::ENTRIES : BetaTraits<7>::ENTRIES) * a.jn) *
                        sizeof(double);
     const void* kern = order == 5 ? reinterpret_cast<const void*>(vjp_kernel<5>)
