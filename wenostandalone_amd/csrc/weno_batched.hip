@@ -30,8 +30,18 @@ constexpr int kBatchThreads = 512;
 constexpr int kBatchMaxNP = 64;
 
 // reconstruct the tile in LDS with order ORDER (after the tile barrier)
+typedef __attribute__((address_space(3))) double LdsMutF64;
+typedef __attribute__((address_space(3))) const long long LdsI64;
+// the descriptors come from memory, so their pointers are generic: casting
+// them to global keeps loads and stores off the flat path (flat accesses
+// count against both wait counters, so LDS waits would include them)
+typedef __attribute__((address_space(1))) double GlbF64;
+typedef __attribute__((address_space(1))) const double GlbCF64;
+
+// tile, table and s_out live in LDS; the address-space-qualified pointers
+// keep their accesses ds_read/ds_write when this function is not inlined
 template <int ORDER, int MODE, bool MP>
-__device__ void batch_compute(const TileArgs& a, const double* tile, double* table, const long long* s_out,
+__device__ void batch_compute(const TileArgs& a, const LdsF64* tile, LdsMutF64* table, const LdsI64* s_out,
                               int lo, int c0, int c1, int jlo, int jhi, bool has_core)
 {
     constexpr int RW = ORDER / 2;
@@ -63,15 +73,15 @@ __device__ void batch_compute(const TileArgs& a, const double* tile, double* tab
     }
     __syncthreads();
     if (!active) return;
-    double* const outL = static_cast<double*>(a.outL);
-    double* const outR = static_cast<double*>(a.outR);
+    GlbF64* const outL = (GlbF64*)a.outL;
+    GlbF64* const outR = (GlbF64*)a.outR;
     const int col = j + a.J0f - lo;
     const BetaRef<NS> B{table + jj, a.jn, NB, NR};
     for (int qq = 0; qq < a.npg; ++qq) {
         const int q = gq * a.npg + qq;
         const long long ob = s_out[q];
         if (ob < 0) continue;
-        const double* row = tile + q * a.ldj + col;
+        const LdsF64* row = tile + q * a.ldj + col;
         if (!core) {
             outL[ob + j] = row[0];
             outR[ob + j] = row[0];
@@ -158,7 +168,7 @@ __global__ void __launch_bounds__(kBatchThreads)
             }
         }
         __syncthreads();
-        const double* const src = static_cast<const double*>(a.src);
+        GlbCF64* const src = (GlbCF64*)a.src;
         for (int e = tid; e < np * nJ; e += T) {
             int q, J;
             if (a.s_j == 1) {
@@ -173,7 +183,9 @@ __global__ void __launch_bounds__(kBatchThreads)
         }
         __syncthreads();
         double* const table = tile + np * ldj;
-#define WM_BATCH(O, M) batch_compute<O, M, MP>(a, tile, table, s_out, lo, c0, c1, jlo, jhi, has_core)
+#define WM_BATCH(O, M)                                                                                      \
+    batch_compute<O, M, MP>(a, lds_ptr(tile), (LdsMutF64*)table, (const LdsI64*)s_out, lo, c0, c1, jlo, jhi, \
+                            has_core)
         if (order == 5) {
             if (mode == MODE_JS) WM_BATCH(5, MODE_JS);
             else if (mode == MODE_TENO) WM_BATCH(5, MODE_TENO);

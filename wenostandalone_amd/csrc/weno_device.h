@@ -123,9 +123,15 @@ __device__ __forceinline__ void load_cf(const Coefs& c, int jc, Cf<7, C>& f)
 // the block's lane groups (they hold the same j): entries r*NS+s for beta,
 // then NR entries of dL and NR of dR.  Reading them at use keeps them out of
 // the register budget.
+// LDS-resident doubles: loads through such a pointer are ds_read (a generic
+// pointer would make them flat loads, which also wait on the wave's
+// outstanding global stores).
+typedef __attribute__((address_space(3))) const double LdsF64;
+__device__ __forceinline__ const LdsF64* lds_ptr(const double* p) { return (const LdsF64*)p; }
+
 template <int NS>
 struct BetaRef {
-    const double* p;
+    const LdsF64* p;
     int stride;
     int nb;   // number of beta entries (NS*NR)
     int nr;   // stencils

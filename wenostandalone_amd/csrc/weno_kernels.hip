@@ -255,7 +255,7 @@ __global__ void __launch_bounds__((BlockTraits<OP, ORDER>::kThreads)) sweep_kern
                 if constexpr (OP == OP_WENO) {
                     // re-read the LDS coefficient table per pencil: an opaque
                     // pointer keeps the loads from being hoisted into registers
-                    const double* bp = beta_lds + jj;
+                    const LdsF64* bp = lds_ptr(beta_lds + jj);
                     if constexpr (!CREG) asm volatile("" : "+v"(bp));
                     const BetaRef<NS> Bl{bp, a.jn, NB, NR};
                     C v[W];

@@ -274,7 +274,7 @@ __global__ void __launch_bounds__(kVjpThreads) vjp_kernel(const TileArgs a, doub
                     const long long ob = s_out[q];
                     double c[W];
                     if (ob >= 0) {
-                        const double* bp = beta_lds + jj;
+                        const LdsF64* bp = lds_ptr(beta_lds + jj);
                         asm volatile("" : "+v"(bp));
                         const BetaRef<NS> Bref{bp, jn, NB, NR};
                         weno_vjp_cell<ORDER>(tile + q * ldj + jj, f, Bref, a.eps, gL[ob + j], gR[ob + j], c);
