@@ -69,7 +69,7 @@ extern "C" int weno_reconstruct_vjp(const weno_grid* g, int order, int dir, int 
     };
     search(256);
     if (T > 256 || best < 0.8) search(tmax);
-    const int entries = (order == 5) ? 15 : 32;
+    const int entries = (order == 5) ? 15 : 33;   // coefficient table row stride
     const int ldj = nJ | 1;
     // LDS per workgroup: input and gradient tiles, the contributions of
     // the outputs (every pencil of the tile for WENO5, one per group for

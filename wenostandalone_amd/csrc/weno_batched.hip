@@ -28,6 +28,7 @@ struct BatchDesc {
 
 constexpr int kBatchThreads = 512;
 constexpr int kBatchMaxNP = 64;
+constexpr int kBatchLdsDescs = 1024;
 
 // reconstruct the tile in LDS with order ORDER (after the tile barrier)
 typedef __attribute__((address_space(3))) double LdsMutF64;
@@ -42,12 +43,13 @@ typedef __attribute__((address_space(1))) const double GlbCF64;
 // keep their accesses ds_read/ds_write when this function is not inlined
 template <int ORDER, int MODE, bool MP>
 __device__ void batch_compute(const TileArgs& a, const LdsF64* tile, LdsMutF64* table, const LdsI64* s_out,
-                              int lo, int c0, int c1, int jlo, int jhi, bool has_core)
+                              int lo, int c0, int c1, int jlo, int jhi, bool has_core, int* counter, int& next)
 {
     constexpr int RW = ORDER / 2;
     constexpr int R = MP ? (RW > 2 ? RW : 2) : RW;
     constexpr int W = 2 * R + 1;
     constexpr int NS = BetaTraits<ORDER>::NS, NB = BetaTraits<ORDER>::N, NR = BetaTraits<ORDER>::NR;
+    constexpr int LDB = BetaTraits<ORDER>::LD;
     const int tid = threadIdx.x;
     const int gq = tid / a.jn;
     const int jj = tid - gq * a.jn;
@@ -63,20 +65,23 @@ __device__ void batch_compute(const TileArgs& a, const LdsF64* tile, LdsMutF64* 
         if (gq == 0) {
             for (int t = 0; t < NB; ++t) {
                 const int r = t / NS, s = t - (t / NS) * NS;
-                table[t * a.jn + jj] = a.c.beta[jc + a.c.nc * (r + 5 * s)];
+                table[jj * LDB + t] = a.c.beta[jc + a.c.nc * (r + 5 * s)];
             }
             for (int r = 0; r < NR; ++r) {
-                table[(NB + r) * a.jn + jj] = a.c.dL[r + 5 * jc];
-                table[(NB + NR + r) * a.jn + jj] = a.c.dR[r + 5 * jc];
+                table[jj * LDB + NB + r] = a.c.dL[r + 5 * jc];
+                table[jj * LDB + NB + NR + r] = a.c.dR[r + 5 * jc];
             }
         }
     }
     __syncthreads();
+    // claim the next item now: its round trip overlaps the reconstruction
+    // (thread 0's coefficient loads have completed, its stores come later)
+    if (threadIdx.x == 0) next = atomicAdd(counter, 1);
     if (!active) return;
     GlbF64* const outL = (GlbF64*)a.outL;
     GlbF64* const outR = (GlbF64*)a.outR;
     const int col = j + a.J0f - lo;
-    const BetaRef<NS> B{table + jj, a.jn, NB, NR};
+    const BetaRef<NS> B{table + jj * LDB, NB, NR};
     for (int qq = 0; qq < a.npg; ++qq) {
         const int q = gq * a.npg + qq;
         const long long ob = s_out[q];
@@ -114,15 +119,25 @@ __global__ void __launch_bounds__(kBatchThreads)
     double* const tile = reinterpret_cast<double*>(smem_raw);
     __shared__ long long s_in[kBatchMaxNP], s_out[kBatchMaxNP];
     __shared__ int s_item, s_desc;
+    // first item of each descriptor, for the item -> descriptor search
+    __shared__ int s_item0[kBatchLdsDescs];
     const int tid = threadIdx.x, T = blockDim.x;
+    const bool lds_search = ndesc <= kBatchLdsDescs;
+    if (lds_search)
+        for (int i = tid; i < ndesc; i += T) s_item0[i] = descs[i].item0;
+    // the next item is claimed while the current one is computed, so the
+    // atomic's round trip is hidden (thread 0 holds it; batch_compute claims)
+    int next = 0;
+    if (tid == 0) next = atomicAdd(counter, 1);
+    __syncthreads();
 
     for (;;) {
         if (tid == 0) {
-            const int it = atomicAdd(counter, 1);
+            const int it = next;
             int lo = 0, hi = ndesc - 1;
             while (lo < hi) {
                 const int mid = (lo + hi + 1) >> 1;
-                if (descs[mid].item0 <= it) lo = mid;
+                if ((lds_search ? s_item0[mid] : descs[mid].item0) <= it) lo = mid;
                 else hi = mid - 1;
             }
             s_item = it;
@@ -185,7 +200,7 @@ __global__ void __launch_bounds__(kBatchThreads)
         double* const table = tile + np * ldj;
 #define WM_BATCH(O, M)                                                                                      \
     batch_compute<O, M, MP>(a, lds_ptr(tile), (LdsMutF64*)table, (const LdsI64*)s_out, lo, c0, c1, jlo, jhi, \
-                            has_core)
+                            has_core, counter, next)
         if (order == 5) {
             if (mode == MODE_JS) WM_BATCH(5, MODE_JS);
             else if (mode == MODE_TENO) WM_BATCH(5, MODE_TENO);

@@ -212,16 +212,22 @@ void plan_tiles(TileArgs& a, Launch& l, int np_override, int chunk_override, int
     if (threads_cap <= 0 && l.op == OP_WENO && l.order == 5) threads_cap = kDefaultWeno5Threads;
     const int tmax = threads_cap > 0 ? std::min(threads_cap, max_threads(l.op, l.order))
                                      : max_threads(l.op, l.order);
-    for (int g = 2; g <= 16; ++g) {
-        const int t = 64 * ((g * jn + 63) / 64);
-        if (t > tmax) break;
-        const double u = double(g * jn) / t;
-        if (u > best + 1e-9) {
-            best = u;
-            G = g;
-            T = t;
+    auto search = [&](int cap) {
+        G = 1;
+        T = 64 * ((jn + 63) / 64);
+        best = double(jn) / T;
+        for (int g = 2; g <= 16; ++g) {
+            const int t = 64 * ((g * jn + 63) / 64);
+            if (t > cap) break;
+            const double u = double(g * jn) / t;
+            if (u > best + 1e-9) {
+                best = u;
+                G = g;
+                T = t;
+            }
         }
-    }
+    };
+    search(tmax);
     const int ldj = std::min(chunk + halo - 1, a.src_len) | 1;
     int npg = std::max(1, (np_target + G - 1) / G);
     npg = std::min(npg, std::max(1, max_np / G));

@@ -119,10 +119,13 @@ __device__ __forceinline__ void load_cf(const Coefs& c, int jc, Cf<7, C>& f)
 }
 
 // Smoothness-indicator coefficients beta_coef(j, r, s) and the linear weights
-// d_cbL/d_cbR(r, j) of one lane, kept in an LDS table [entry][lane] shared by
+// d_cbL/d_cbR(r, j) of one lane, kept in an LDS table [lane][entry] shared by
 // the block's lane groups (they hold the same j): entries r*NS+s for beta,
-// then NR entries of dL and NR of dR.  Reading them at use keeps them out of
-// the register budget.
+// then NR entries of dL and NR of dR, rows padded to an odd number of doubles
+// (BetaTraits::LD) so that the lanes of a wave hit distinct banks.  A lane's
+// entries sit at constant offsets from its row, so they are read with
+// immediate-offset ds_read2_b64 and no address arithmetic.  Reading them at
+// use keeps them out of the register budget.
 // LDS-resident doubles: loads through such a pointer are ds_read (a generic
 // pointer would make them flat loads, which also wait on the wave's
 // outstanding global stores).
@@ -131,13 +134,12 @@ __device__ __forceinline__ const LdsF64* lds_ptr(const double* p) { return (cons
 
 template <int NS>
 struct BetaRef {
-    const LdsF64* p;
-    int stride;
-    int nb;   // number of beta entries (NS*NR)
-    int nr;   // stencils
-    __device__ __forceinline__ double operator()(int r, int s) const { return p[(r * NS + s) * stride]; }
-    __device__ __forceinline__ double dl(int r) const { return p[(nb + r) * stride]; }
-    __device__ __forceinline__ double dr(int r) const { return p[(nb + nr + r) * stride]; }
+    const LdsF64* p;   // the lane's row
+    int nb;            // number of beta entries (NS*NR)
+    int nr;            // stencils
+    __device__ __forceinline__ double operator()(int r, int s) const { return p[r * NS + s]; }
+    __device__ __forceinline__ double dl(int r) const { return p[nb + r]; }
+    __device__ __forceinline__ double dr(int r) const { return p[nb + nr + r]; }
 };
 template <int ORDER>
 struct BetaTraits {
@@ -145,6 +147,7 @@ struct BetaTraits {
     static constexpr int NR = ORDER == 3 ? 2 : (ORDER == 5 ? 3 : 4);   // stencils
     static constexpr int N = NS * NR;
     static constexpr int ENTRIES = N + 2 * NR;
+    static constexpr int LD = (ENTRIES % 2) ? ENTRIES : ENTRIES + 1;   // table row stride
 };
 
 // The same table held in registers (loaded once per lane); fits the

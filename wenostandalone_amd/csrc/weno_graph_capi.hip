@@ -48,7 +48,8 @@ struct GraphHandle {
     std::vector<void*> buffers;   // device memory owned by the graph
 };
 
-int entries_of(int order) { return order == 3 ? 6 : (order == 5 ? 15 : 32); }
+// coefficient table row stride (BetaTraits::LD) per order
+int entries_of(int order) { return order == 3 ? 7 : (order == 5 ? 15 : 33); }
 
 int plan_batch(const weno_block* blocks, int nblocks, int dir, int flags, int threads, int np_target,
                BatchPlan& plan)

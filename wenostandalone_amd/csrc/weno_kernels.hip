@@ -43,12 +43,15 @@ struct OrderTraits {
     static constexpr int R = ORDER / 2;
 };
 
-// WENO7 keeps 56 coefficients per lane in registers: give its waves the
-// 256-VGPR budget of two waves per SIMD (512-thread blocks); everything else
-// runs four waves per SIMD (1024-thread blocks, 128 VGPRs).
-template <int OP, int ORDER>
+// WENO7 FP64 needs more than 128 VGPRs: its blocks are 512 threads and its
+// waves may use up to 256 VGPRs; FP32/BF16 WENO7 is held to 128 VGPRs (two
+// 512-thread blocks per CU); everything else runs four waves per SIMD
+// (1024-thread blocks, 128 VGPRs).
+template <int OP, int ORDER, typename TIN = double>
 struct BlockTraits {
     static constexpr int kThreads = (OP == OP_WENO && ORDER == 7) ? 512 : kMaxThreads;
+    // second __launch_bounds__ argument: minimum waves per SIMD (4 -> 128 VGPRs)
+    static constexpr int kMinWaves = (OP == OP_WENO && ORDER == 7 && sizeof(TIN) < sizeof(double)) ? 4 : 1;
 };
 
 // ---------------------------------------------------------------------------
@@ -63,7 +66,8 @@ struct BlockTraits {
 //   tile.  Pencil address tables are built two tiles ahead (3 slots).
 // ---------------------------------------------------------------------------
 template <int OP, int ORDER, int MODE, bool MP, typename TIN, typename TOUT, bool CREG>
-__global__ void __launch_bounds__((BlockTraits<OP, ORDER>::kThreads)) sweep_kernel(const TileArgs a, const int nt)
+__global__ void __launch_bounds__((BlockTraits<OP, ORDER, TIN>::kThreads), (BlockTraits<OP, ORDER, TIN>::kMinWaves))
+    sweep_kernel(const TileArgs a, const int nt)
 {
     using C = typename CT<TIN>::type;
     constexpr int RW = (OP == OP_WENO) ? OrderTraits<ORDER>::R : 0;
@@ -84,6 +88,7 @@ __global__ void __launch_bounds__((BlockTraits<OP, ORDER>::kThreads)) sweep_kern
     constexpr int NS = BetaTraits<OB>::NS;
     constexpr int NB = (OP == OP_WENO) ? BetaTraits<OB>::N : 0;
     constexpr int NR = BetaTraits<OB>::NR;
+    constexpr int LDB = BetaTraits<OB>::LD;
     double* const beta_lds =
         reinterpret_cast<double*>(smem_raw + ((2 * static_cast<size_t>(np) * ldj * sizeof(C) + 15) & ~size_t(15)));
 
@@ -165,12 +170,12 @@ __global__ void __launch_bounds__((BlockTraits<OP, ORDER>::kThreads)) sweep_kern
 #pragma unroll
                 for (int t = 0; t < NB; ++t) {
                     const int r = t / NS, s = t - (t / NS) * NS;
-                    beta_lds[t * a.jn + jj] = a.c.beta[jc + a.c.nc * (r + 5 * s)];
+                    beta_lds[jj * LDB + t] = a.c.beta[jc + a.c.nc * (r + 5 * s)];
                 }
 #pragma unroll
                 for (int r = 0; r < NR; ++r) {
-                    beta_lds[(NB + r) * a.jn + jj] = a.c.dL[r + 5 * jc];
-                    beta_lds[(NB + NR + r) * a.jn + jj] = a.c.dR[r + 5 * jc];
+                    beta_lds[jj * LDB + NB + r] = a.c.dL[r + 5 * jc];
+                    beta_lds[jj * LDB + NB + NR + r] = a.c.dR[r + 5 * jc];
                 }
             }
         }
@@ -255,9 +260,9 @@ __global__ void __launch_bounds__((BlockTraits<OP, ORDER>::kThreads)) sweep_kern
                 if constexpr (OP == OP_WENO) {
                     // re-read the LDS coefficient table per pencil: an opaque
                     // pointer keeps the loads from being hoisted into registers
-                    const LdsF64* bp = lds_ptr(beta_lds + jj);
+                    const LdsF64* bp = lds_ptr(beta_lds + jj * LDB);
                     if constexpr (!CREG) asm volatile("" : "+v"(bp));
-                    const BetaRef<NS> Bl{bp, a.jn, NB, NR};
+                    const BetaRef<NS> Bl{bp, NB, NR};
                     C v[W];
 #pragma unroll
                     for (int o = 0; o < W; ++o) v[o] = row[o - R];
@@ -326,7 +331,7 @@ static hipError_t go(TileArgs a, const Launch& l, hipStream_t st)
 {
     using C = typename CT<TIN>::type;
     auto kern = sweep_kernel<OP, ORDER, MODE, MP, TIN, TOUT, CREG>;
-    constexpr int NB = (OP == OP_WENO && !CREG) ? BetaTraits<ORDER>::ENTRIES : 0;
+    constexpr int NB = (OP == OP_WENO && !CREG) ? BetaTraits<ORDER>::LD : 0;
     const size_t lds = ((2 * static_cast<size_t>(a.np) * a.ldj * sizeof(C) + 15) & ~size_t(15)) +
                        static_cast<size_t>(NB) * a.jn * sizeof(double);
     if (lds > 64 * 1024) {

@@ -179,6 +179,7 @@ __global__ void __launch_bounds__(kVjpThreads) vjp_kernel(const TileArgs a, doub
     constexpr int W = 2 * R + 1;
     constexpr int NS = BetaTraits<ORDER>::NS;
     constexpr int NB = BetaTraits<ORDER>::N;
+    constexpr int LDB = BetaTraits<ORDER>::LD;
     constexpr int NR = BetaTraits<ORDER>::NR;
 
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -216,11 +217,11 @@ __global__ void __launch_bounds__(kVjpThreads) vjp_kernel(const TileArgs a, doub
         if (gq == 0) {
             for (int t = 0; t < NB; ++t) {
                 const int r = t / NS, s = t - (t / NS) * NS;
-                beta_lds[t * jn + jj] = a.c.beta[jc + a.c.nc * (r + 5 * s)];
+                beta_lds[jj * LDB + t] = a.c.beta[jc + a.c.nc * (r + 5 * s)];
             }
             for (int r = 0; r < NR; ++r) {
-                beta_lds[(NB + r) * jn + jj] = a.c.dL[r + 5 * jc];
-                beta_lds[(NB + NR + r) * jn + jj] = a.c.dR[r + 5 * jc];
+                beta_lds[jj * LDB + NB + r] = a.c.dL[r + 5 * jc];
+                beta_lds[jj * LDB + NB + NR + r] = a.c.dR[r + 5 * jc];
             }
         }
     }
@@ -274,9 +275,9 @@ __global__ void __launch_bounds__(kVjpThreads) vjp_kernel(const TileArgs a, doub
                     const long long ob = s_out[q];
                     double c[W];
                     if (ob >= 0) {
-                        const LdsF64* bp = lds_ptr(beta_lds + jj);
+                        const LdsF64* bp = lds_ptr(beta_lds + jj * LDB);
                         asm volatile("" : "+v"(bp));
-                        const BetaRef<NS> Bref{bp, jn, NB, NR};
+                        const BetaRef<NS> Bref{bp, NB, NR};
                         weno_vjp_cell<ORDER>(tile + q * ldj + jj, f, Bref, a.eps, gL[ob + j], gR[ob + j], c);
                     } else {
 #pragma unroll
@@ -329,7 +330,7 @@ hipError_t launch_vjp(TileArgs a, int order, double* grad, hipStream_t st)
 {
     const size_t lds = (2 * static_cast<size_t>(a.np) * a.ldj +
                         static_cast<size_t>(order == 5 ? a.np : a.groups) * order * a.jn +
-                        static_cast<size_t>(order == 5 ? BetaTraits<5>::ENTRIES : BetaTraits<7>::ENTRIES) * a.jn) *
+                        static_cast<size_t>(order == 5 ? BetaTraits<5>::LD : BetaTraits<7>::LD) * a.jn) *
                        sizeof(double);
     const void* kern = order == 5 ? reinterpret_cast<const void*>(vjp_kernel<5>)
                                   : reinterpret_cast<const void*>(vjp_kernel<7>);
