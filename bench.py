@@ -346,6 +346,94 @@ def suite(path):
     print(json.dumps(report), file=sys.stderr, flush=True)
 
 
+def families(path, m=199, v=8, buff=6, reps=10):
+    """Every kernel family at m+1 cubed x v FP64 (default 200^3 x 8), x, y and
+    z sweeps: HIP-event time per launch on the library stream and its
+    algorithmic HBM bytes (each distinct input element read once, each output
+    written once), as a fraction of the 8 TB/s HBM peak.  Per direction with
+    ns = nk = nl = m+1+2*buff (212), nc = ns-6 (206):
+      reshape .................. read (ns+6) nk nl v, write the same
+      order-1 copy ............. read ns nk nl v, write 2 ns nk nl v
+      WENO3, fused copy ........ read (ns+2) nk nl v, write 2 ns nk nl v
+      WENO5/7, fused copy (+MP). read ns nk nl v, write 2 ns nk nl v
+      WENO5/7, core only ....... read (nc+2R) nk nl v, write 2 nc nk nl v
+      MP standalone ............ read (nc+4) nk nl v of v_rs and vL, vR on
+                                 the core, write vL, vR on the core
+    WENO forms run from v_vf (direction-aware loads); MP reads the reshaped
+    array as the reference's s_preserve_monotonicity does (:1544)."""
+    import wenostandalone_amd as wm
+    from wenostandalone_amd import fields
+    from wenostandalone_amd.device import DeviceArray
+
+    wm.check(wm.lib.weno_init(0))
+    h = hip()
+    nx = m + 1 + 2 * buff
+    nc = nx - 6
+    plane = nx * nx * v * 8           # bytes of one j-slab of a face array
+    f = fields.seeded_field(m, m, m, v, buff, seed=0)
+    vf = DeviceArray.from_host(f)
+    del f
+    rs = DeviceArray((nx + 6, nx, nx, v))
+    L, R = DeviceArray((nx, nx, nx, v)), DeviceArray((nx, nx, nx, v))
+    coefs = []
+    for d in range(3):
+        devs = [DeviceArray.from_host(c) for c in fields.harness_coefficients(nc, fields.DIR_OFFSETS[d])]
+        coefs.append((devs, wm.Coeffs(*[x.ptr for x in devs])))
+    branches = {"js": {}, "teno": {"wenojs": False, "teno": True},
+                "mapped": {"wenojs": False, "mapped_weno": True}, "wenoz": {"wenojs": False, "wenoz": True}}
+    grids = {k: wm.make_grid(m, v=v, buff=buff, **kw) for k, kw in branches.items()}
+    g = grids["js"]
+
+    forms = [("reshape", lambda d: wm.lib.weno_reshape(C.byref(g), d, wm.F64, vf.vp, rs.vp),
+              2 * (nx + 6) * plane),
+             ("order1_copy", lambda d: wm.lib.weno_order1_copy(C.byref(g), d, wm.F64, vf.vp, L.vp, R.vp),
+              3 * nx * plane)]
+
+    def recon(order, br, flags):
+        return lambda d: wm.lib.weno_reconstruct(C.byref(grids[br]), order, d, wm.SRC_FIELD, wm.F64, vf.vp,
+                                                 C.byref(coefs[d][1]), L.vp, R.vp, flags)
+    for order in (3, 5, 7):
+        for br in branches:
+            if order == 3 and br == "teno":
+                continue      # the reference has no TENO3 branch
+            rd = nx + 2 if order == 3 else nx
+            forms.append((f"weno{order}_{br}_fused_copy", recon(order, br, wm.FUSE_COPY), (rd + 2 * nx) * plane))
+    for order in (5, 7):
+        forms.append((f"weno{order}_js_core_only", recon(order, "js", 0), (nc + order - 1 + 2 * nc) * plane))
+        forms.append((f"weno{order}_js_fused_copy_mp", recon(order, "js", wm.FUSE_COPY | wm.FUSE_MP),
+                      3 * nx * plane))
+    forms.append(("mp_standalone", lambda d: wm.lib.weno_mp_limit(C.byref(g), d, wm.SRC_RESHAPED, wm.F64, rs.vp,
+                                                                 L.vp, R.vp), (nc + 4 + 4 * nc) * plane))
+
+    wm.check(wm.lib.weno_set_async(1))
+    stream = wm.lib.weno_get_stream()
+    out = {"workload": f"{m + 1}^3 x {v} FP64, harness coefficients, seeded field recipe 0",
+           "peak_GBs": HBM_PEAK_GBS, "forms": {}}
+    for name, fn, nbytes in forms:
+        row = {"bytes_per_launch": nbytes}
+        for d, ax in enumerate("xyz"):
+            if name == "mp_standalone":
+                # the reshaped array of this direction and faces from WENO5 on it
+                wm.check(wm.lib.weno_reshape(C.byref(g), d, wm.F64, vf.vp, rs.vp))
+                wm.check(recon(5, "js", wm.FUSE_COPY)(d))
+            ms = _time(h, stream, lambda: wm.check(fn(d)), reps=reps)
+            row[f"{ax}_us"] = ms * 1e3
+            row[f"{ax}_GBs"] = nbytes / (ms * 1e-3) / 1e9
+        mean_us = float(np.mean([row[f"{ax}_us"] for ax in "xyz"]))
+        row["mean_us"] = mean_us
+        row["GBs"] = nbytes / (mean_us * 1e-6) / 1e9
+        row["frac_of_peak"] = row["GBs"] / HBM_PEAK_GBS
+        out["forms"][name] = row
+        print(f"families: {name:28s} {mean_us:9.1f} us  {row['GBs']:7.0f} GB/s  {row['frac_of_peak']:.3f}",
+              file=sys.stderr, flush=True)
+    wm.check(wm.lib.weno_set_async(0))
+    for x in (vf, rs, L, R):
+        x.free()
+    with open(path, "w") as fh:
+        json.dump(out, fh, indent=1)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -355,6 +443,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--flags", type=int, default=0, help="extra weno_reconstruct flags")
     ap.add_argument("--suite", default=None, help="also run the secondary measurements into this JSON file")
+    ap.add_argument("--families", default=None,
+                    help="time every kernel family against its roofline into this JSON file, then exit")
     ap.add_argument("--copy-yardstick", action="store_true",
                     help="run one plain streaming copy of a face array first (counter calibration)")
     ap.add_argument("--record-traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
@@ -367,6 +457,10 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (the sweep would shard "
               "by sub-domain with a buff_size halo exchange; see DESIGN.md).", file=sys.stderr)
         sys.exit(2)
+
+    if args.families:
+        families(args.families, m=args.m)
+        return
 
     import wenostandalone_amd as wm
     from wenostandalone_amd import fields

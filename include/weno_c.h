@@ -57,10 +57,13 @@ enum { WENO_F64 = 0, WENO_F32 = 1, WENO_BF16 = 2 };
 #define WENO_ONLY_R 16     /* write vR only (s_weno5_right_kernel, :1272) */
 
 /* Grid and scheme parameters: namelist &grid_params (weno_standalone.f90:
- * 57-66, :117-118) plus `teno`, a compile-time parameter in the reference
- * (:74) and a runtime switch here.  wenojs takes precedence over teno
- * (:552, :1475); with neither, WENO outputs are NaN exactly as in the
- * reference (no weight branch taken). */
+ * 57-66, :117-118) plus `teno`, `mapped_weno` and `wenoz`, compile-time
+ * parameters in the reference (:72-74) and runtime switches here.  The
+ * weight branch follows the reference's precedence: wenojs, then
+ * mapped_weno, then wenoz, then teno (:552-606, :1475-1503); WENO3 has no
+ * TENO branch; with no branch taken the outputs are NaN exactly as in the
+ * reference.  WENO-Z uses (tau/beta) to the power 1 for WENO3/5 and
+ * wenoz_q = 2 for WENO7 (:75), as the reference writes them. */
 typedef struct weno_grid {
     int m_cells, n_cells, p_cells;
     int v_size;
@@ -69,6 +72,8 @@ typedef struct weno_grid {
     int wenojs;
     int teno;
     double teno_CT;
+    int mapped_weno;
+    int wenoz;
 } weno_grid;
 
 /* One direction's coefficient set (device or present host addresses). */

@@ -24,7 +24,8 @@ REF_PROGRAM = os.path.join(REF_DIR, "weno_standalone")
 
 class OGrid(C.Structure):
     _fields_ = [("m", C.c_int), ("n", C.c_int), ("p", C.c_int), ("buff", C.c_int), ("v", C.c_int),
-                ("eps", C.c_double), ("wenojs", C.c_int), ("teno", C.c_int), ("teno_ct", C.c_double)]
+                ("eps", C.c_double), ("wenojs", C.c_int), ("teno", C.c_int), ("teno_ct", C.c_double),
+                ("mapped_weno", C.c_int), ("wenoz", C.c_int)]
 
 
 _lib = None
@@ -57,10 +58,12 @@ def lib():
     return _lib
 
 
-def grid(m, n=None, p=None, v=8, buff=6, eps=1e-16, wenojs=True, teno=False, teno_ct=1e-5) -> OGrid:
+def grid(m, n=None, p=None, v=8, buff=6, eps=1e-16, wenojs=True, teno=False, teno_ct=1e-5,
+         mapped_weno=False, wenoz=False) -> OGrid:
     n = m if n is None else n
     p = m if p is None else p
-    return OGrid(m, n, p, buff, v, eps, int(bool(wenojs)), int(bool(teno)), teno_ct)
+    return OGrid(m, n, p, buff, v, eps, int(bool(wenojs)), int(bool(teno)), teno_ct,
+                 int(bool(mapped_weno)), int(bool(wenoz)))
 
 
 def _p(a: np.ndarray):
@@ -159,11 +162,26 @@ def have_ref() -> bool:
     return os.path.exists(REF_DRIVER)
 
 
-def run_ref(mode: str, m: int, v: int, buff: int, field: np.ndarray | None, eps=1e-16, wenojs=True):
+BRANCHES = ("teno", "wenoz", "mapped_weno")
+
+
+def ref_driver(branch: str | None = None) -> str:
+    """The reference oracle driver; `branch` names the build with that
+    compile-time weight parameter set (teno, wenoz, mapped_weno)."""
+    if branch is None:
+        return REF_DRIVER
+    if branch not in BRANCHES:
+        raise ValueError(f"unknown weight branch {branch!r}")
+    return os.path.join(REF_DIR, branch, "weno_ref_driver")
+
+
+def run_ref(mode: str, m: int, v: int, buff: int, field: np.ndarray | None, eps=1e-16, wenojs=True,
+            branch: str | None = None):
     """Runs the reference module's kernel sequence `mode` (cubic grid m=n=p)
     on `field` (None = the harness field) and returns the six output arrays
     (vL_x, vR_x, vL_y, vR_y, vL_z, vR_z) or, for mode='reshape', the three
-    reshaped arrays."""
+    reshaped arrays.  `branch` selects a reference build with one weight
+    parameter set (see ref_driver); pass wenojs=False to reach it."""
     nx = m + 1 + 2 * buff
     with tempfile.TemporaryDirectory() as td:
         fin = "harness"
@@ -175,7 +193,7 @@ def run_ref(mode: str, m: int, v: int, buff: int, field: np.ndarray | None, eps=
         # threads of its `parallel do` (weno_standalone.f90:1550-1559), so a
         # multi-threaded host run races
         env = dict(os.environ, OMP_TARGET_OFFLOAD="DISABLED", OMP_NUM_THREADS="1")
-        subprocess.run([REF_DRIVER, mode, str(m), str(m), str(m), str(v), str(buff), repr(float(eps)),
+        subprocess.run([ref_driver(branch), mode, str(m), str(m), str(m), str(v), str(buff), repr(float(eps)),
                         "T" if wenojs else "F", fin, fout], check=True, env=env,
                        stdout=subprocess.DEVNULL)
         raw = np.fromfile(fout, dtype=np.float64)

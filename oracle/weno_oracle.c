@@ -12,9 +12,13 @@
  *   weno7 / weno3 (core) / mp / checksum: bitwise against oracle/_ref on seeded
  *   inputs (committed fixtures) + tests/golden/weno5.txt checksums + closed
  *   forms (constant and linear fields, MP in-bounds idempotence).
- *   TENO (weno5/weno7): the reference fixes teno=.false. at compile time, so no
- *   reference output exists; pinned by hand-derived closed forms on crafted
- *   stencils (tests/test_oracle.py::test_teno_*).
+ *   TENO (weno5/weno7), WENO-Z and mapped WENO (weno3/weno5/weno7): the
+ *   reference selects them with compile-time parameters (teno, wenoz,
+ *   mapped_weno, :72-74), all .false. as shipped.  oracle/Makefile builds the
+ *   reference once per branch with that one parameter set to .true.
+ *   (oracle/_ref/<name>/); the committed fixtures tests/golden/ref_branches.npz
+ *   hold its outputs, compared bitwise.  TENO is also pinned by hand-derived
+ *   closed forms on crafted stencils (tests/test_oracle.py::test_teno_*).
  *   weno3 halo cells (j outside the core range): the reference reads its
  *   coefficient arrays out of bounds there (:395-399 vs :413), so nothing
  *   pins them; this oracle clamps the coefficient index to the core range
@@ -141,6 +145,30 @@ void oracle_order1_copy(const oracle_grid* g, int dir, const double* v_vf,
                 }
 }
 
+/* The mapping of mapped WENO (weno_standalone.f90:436-437, :561-569,
+ * :1481-1482): alpha_r <- (d_r (1 + d_r - 3 w_r) + w_r**2) *
+ * (w_r / (d_r**2 + w_r (1 - 2 d_r))), with w = alpha / sum(alpha) taken over
+ * the stencils in array order.  Real exponent 2 is a product (x*x). */
+static inline double map_one(double d, double w)
+{
+    return (d * (1.0 + d - 3.0 * w) + w * w) * (w / (d * d + w * (1.0 - 2.0 * d)));
+}
+static void map2(const double* d, double* a0, double* a1)
+{
+    double s = *a0 + *a1;
+    double w0 = *a0 / s, w1 = *a1 / s;
+    *a0 = map_one(d[0], w0);
+    *a1 = map_one(d[1], w1);
+}
+static void map3(const double* d, double* a0, double* a1, double* a2)
+{
+    double s = *a0 + *a1 + *a2;
+    double w0 = *a0 / s, w1 = *a1 / s, w2 = *a2 / s;
+    *a0 = map_one(d[0], w0);
+    *a1 = map_one(d[1], w1);
+    *a2 = map_one(d[2], w2);
+}
+
 #define RS(J) v_rs[(size_t)(J) + base_in]
 
 /* weno_standalone.f90:392-475 (WENO3; j runs over the whole `is` range :413).
@@ -172,9 +200,20 @@ void oracle_weno3(const oracle_grid* g, int dir, const double* v_rs,
                     double poly1 = RS(J) + polyL[jc + nc * 1] * dvdm1;
                     double beta0 = beta[jc + nc * 0] * dvd0 * dvd0 + eps;
                     double beta1 = beta[jc + nc * 1] * dvdm1 * dvdm1 + eps;
+                    double tau = 0.0;
                     if (g->wenojs) {
                         alpha0 = dL[0 + 5 * jc] / (beta0 * beta0);
                         alpha1 = dL[1 + 5 * jc] / (beta1 * beta1);
+                    } else if (g->mapped_weno) {
+                        /* :433-437 */
+                        alpha0 = dL[0 + 5 * jc] / (beta0 * beta0);
+                        alpha1 = dL[1 + 5 * jc] / (beta1 * beta1);
+                        map2(dL + 5 * jc, &alpha0, &alpha1);
+                    } else if (g->wenoz) {
+                        /* :439-441 */
+                        tau = fabs(beta1 - beta0);
+                        alpha0 = dL[0 + 5 * jc] * (1.0 + tau / beta0);
+                        alpha1 = dL[1 + 5 * jc] * (1.0 + tau / beta1);
                     }
                     double sum_alpha = alpha0 + alpha1;
                     omega0 = alpha0 / sum_alpha;
@@ -186,6 +225,15 @@ void oracle_weno3(const oracle_grid* g, int dir, const double* v_rs,
                     if (g->wenojs) {
                         alpha0 = dR[0 + 5 * jc] / (beta0 * beta0);
                         alpha1 = dR[1 + 5 * jc] / (beta1 * beta1);
+                    } else if (g->mapped_weno) {
+                        /* :456-460 */
+                        alpha0 = dR[0 + 5 * jc] / (beta0 * beta0);
+                        alpha1 = dR[1 + 5 * jc] / (beta1 * beta1);
+                        map2(dR + 5 * jc, &alpha0, &alpha1);
+                    } else if (g->wenoz) {
+                        /* :462-463, tau of the left side */
+                        alpha0 = dR[0 + 5 * jc] * (1.0 + tau / beta0);
+                        alpha1 = dR[1 + 5 * jc] * (1.0 + tau / beta1);
                     }
                     sum_alpha = alpha0 + alpha1;
                     omega0 = alpha0 / sum_alpha;
@@ -231,11 +279,23 @@ void oracle_weno5(const oracle_grid* g, int dir, const double* v_rs,
                                    + BC(1, 2) * dvd_m1 * dvd_m1 + eps;
                     double beta2 = BC(2, 0) * dvd_m1 * dvd_m1 + BC(2, 1) * dvd_m1 * dvd_m2
                                    + BC(2, 2) * dvd_m2 * dvd_m2 + eps;
-                    double sum_alpha, omega0, omega1, omega2, tau;
+                    double sum_alpha, omega0, omega1, omega2, tau = 0.0;
                     if (g->wenojs) {
                         alpha0 = dL[0 + 5 * jc] / (beta0 * beta0);
                         alpha1 = dL[1 + 5 * jc] / (beta1 * beta1);
                         alpha2 = dL[2 + 5 * jc] / (beta2 * beta2);
+                    } else if (g->mapped_weno) {
+                        /* :557-570 */
+                        alpha0 = dL[0 + 5 * jc] / (beta0 * beta0);
+                        alpha1 = dL[1 + 5 * jc] / (beta1 * beta1);
+                        alpha2 = dL[2 + 5 * jc] / (beta2 * beta2);
+                        map3(dL + 5 * jc, &alpha0, &alpha1, &alpha2);
+                    } else if (g->wenoz) {
+                        /* :572-576 */
+                        tau = fabs(beta2 - beta0);
+                        alpha0 = dL[0 + 5 * jc] * (1.0 + (tau / beta0));
+                        alpha1 = dL[1 + 5 * jc] * (1.0 + (tau / beta1));
+                        alpha2 = dL[2 + 5 * jc] * (1.0 + (tau / beta2));
                     } else if (g->teno) {
                         /* :578-606 */
                         tau = fabs(beta2 - beta0);
@@ -266,6 +326,17 @@ void oracle_weno5(const oracle_grid* g, int dir, const double* v_rs,
                         alpha0 = dR[0 + 5 * jc] / (beta0 * beta0);
                         alpha1 = dR[1 + 5 * jc] / (beta1 * beta1);
                         alpha2 = dR[2 + 5 * jc] / (beta2 * beta2);
+                    } else if (g->mapped_weno) {
+                        /* :632-645 */
+                        alpha0 = dR[0 + 5 * jc] / (beta0 * beta0);
+                        alpha1 = dR[1 + 5 * jc] / (beta1 * beta1);
+                        alpha2 = dR[2 + 5 * jc] / (beta2 * beta2);
+                        map3(dR + 5 * jc, &alpha0, &alpha1, &alpha2);
+                    } else if (g->wenoz) {
+                        /* :647-650, tau of the left side */
+                        alpha0 = dR[0 + 5 * jc] * (1.0 + (tau / beta0));
+                        alpha1 = dR[1 + 5 * jc] * (1.0 + (tau / beta1));
+                        alpha2 = dR[2 + 5 * jc] * (1.0 + (tau / beta2));
                     } else if (g->teno) {
                         alpha0 = delta0 * dR[0 + 5 * jc];
                         alpha1 = delta1 * dR[1 + 5 * jc];
@@ -319,14 +390,32 @@ void oracle_weno7(const oracle_grid* g, int dir, const double* v_rs,
                                + BC(2, 3) * dm1 * dm1 + BC(2, 4) * dm1 * dm2 + BC(2, 5) * dm2 * dm2 + eps;
                     beta_[0] = BC(3, 0) * dm1 * dm1 + BC(3, 1) * dm1 * dm2 + BC(3, 2) * dm1 * dm3
                                + BC(3, 3) * dm2 * dm2 + BC(3, 4) * dm2 * dm3 + BC(3, 5) * dm3 * dm3 + eps;
+                    double tau = 0.0;
                     if (g->wenojs) {
                         for (int q = 0; q < 4; ++q) alpha[q] = dL[q + 5 * jc] / (beta_[q] * beta_[q]);
+                    } else if (g->mapped_weno) {
+                        /* :1478-1482: the sum runs over all five entries (alpha(4) = 0) */
+                        for (int q = 0; q < 4; ++q) alpha[q] = dL[q + 5 * jc] / (beta_[q] * beta_[q]);
+                        double s = alpha[0] + alpha[1] + alpha[2] + alpha[3] + alpha[4];
+                        for (int q = 0; q < 5; ++q) omega[q] = alpha[q] / s;
+                        for (int q = 0; q < 4; ++q) alpha[q] = map_one(dL[q + 5 * jc], omega[q]);
+                    } else if (g->wenoz) {
+                        /* :1484-1488, wenoz_q = 2 */
+                        tau = fabs(beta_[3] - beta_[0]);
+                        for (int q = 0; q < 4; ++q) {
+                            double t = tau / beta_[q];
+                            alpha[q] = dL[q + 5 * jc] * (1.0 + t * t);
+                        }
                     } else if (g->teno) {
                         /* :1490-1503 */
-                        double tau = fabs(beta_[4] - beta_[3]);
+                        tau = fabs(beta_[4] - beta_[3]);
                         for (int q = 0; q < 5; ++q) alpha[q] = 1.0 + tau / beta_[q];
+                        /* (alpha**3._wp)**2._wp: the reference build evaluates
+                         * the real power 3 as alpha*alpha*alpha (bitwise
+                         * against oracle/_ref/teno) and the power 6 of
+                         * WENO5 (:580) as pow */
                         for (int q = 0; q < 5; ++q) {
-                            double t = pow(alpha[q], 3.0);
+                            double t = alpha[q] * alpha[q] * alpha[q];
                             alpha[q] = t * t;
                         }
                         double s = alpha[0] + alpha[1] + alpha[2] + alpha[3] + alpha[4];
@@ -346,6 +435,18 @@ void oracle_weno7(const oracle_grid* g, int dir, const double* v_rs,
                     poly[0] = vj + PR(3, 0) * dm1 + PR(3, 1) * dm2 + PR(3, 2) * dm3;
                     if (g->wenojs) {
                         for (int q = 0; q < 4; ++q) alpha[q] = dR[q + 5 * jc] / (beta_[q] * beta_[q]);
+                    } else if (g->mapped_weno) {
+                        /* :1517-1521 */
+                        for (int q = 0; q < 4; ++q) alpha[q] = dR[q + 5 * jc] / (beta_[q] * beta_[q]);
+                        double s2 = alpha[0] + alpha[1] + alpha[2] + alpha[3] + alpha[4];
+                        for (int q = 0; q < 5; ++q) omega[q] = alpha[q] / s2;
+                        for (int q = 0; q < 4; ++q) alpha[q] = map_one(dR[q + 5 * jc], omega[q]);
+                    } else if (g->wenoz) {
+                        /* :1523-1526, tau of the left side */
+                        for (int q = 0; q < 4; ++q) {
+                            double t = tau / beta_[q];
+                            alpha[q] = dR[q + 5 * jc] * (1.0 + t * t);
+                        }
                     } else if (g->teno) {
                         for (int q = 0; q < 4; ++q) alpha[q] = delta[q] * dR[q + 5 * jc];
                     }

@@ -33,6 +33,8 @@ typedef struct oracle_grid {
     int wenojs;           /* wenojs  */
     int teno;             /* teno (a compile-time parameter in the reference) */
     double teno_ct;       /* teno_CT */
+    int mapped_weno;      /* mapped_weno (a compile-time parameter, :72) */
+    int wenoz;            /* wenoz (a compile-time parameter, :73; wenoz_q = 2, :75) */
 } oracle_grid;
 
 /* extents: sweep length ns, transverse nk, nl for direction dir (0=x,1=y,2=z) */

@@ -8,6 +8,12 @@ kernels, built from its source by `make -C oracle ref` (this container only).
                    (order-1 copy, then the kernel, then MP for mp*).
                    weno3 halo cells are stored but are undefined in the
                    reference (out-of-bounds coefficient reads).
+  ref_branches.npz the reference's TENO, WENO-Z and mapped-WENO weights: the
+                   reference built with its compile-time parameter teno,
+                   wenoz or mapped_weno set (oracle/_ref/<branch>/) and run
+                   with wenojs=F, m=n=p=3, v_size=1 (seeded recipe 2), weno3
+                   (not TENO: the reference has no TENO3 branch) / weno5 /
+                   weno7, core range of vL/vR for x, y and z.
   ref_checksums.json  X/Y/Z checksums (sum|vL|+sum|vR|) of the reference at
                    m=n=p=31, v_size=8, for the harness field and for a seeded
                    field, per kernel sequence.
@@ -40,6 +46,17 @@ def main():
             out[f"{mode}_{name}_L"] = r[2 * d]
             out[f"{mode}_{name}_R"] = r[2 * d + 1]
     np.savez_compressed(os.path.join(HERE, "ref_small.npz"), **out)
+
+    m, v = 3, 1
+    f = fields.seeded_field(m, m, m, v, b, seed=13, recipe=2)
+    out = {"field": f, "m": m, "v": v, "buff": b}
+    for branch in oracle.BRANCHES:
+        for mode in (["weno5", "weno7"] if branch == "teno" else ["weno3", "weno5", "weno7"]):
+            r = oracle.run_ref(mode, m, v, b, f, wenojs=False, branch=branch)
+            for d, name in enumerate("xyz"):
+                out[f"{branch}_{mode}_{name}_L"] = np.ascontiguousarray(r[2 * d][3:-3])
+                out[f"{branch}_{mode}_{name}_R"] = np.ascontiguousarray(r[2 * d + 1][3:-3])
+    np.savez_compressed(os.path.join(HERE, "ref_branches.npz"), **out)
 
     sums = {}
     m, v = 31, 8

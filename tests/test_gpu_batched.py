@@ -31,7 +31,7 @@ def _init():
 class Blocks:
     """Host and device data of a set of independent blocks."""
 
-    def __init__(self, specs, seed):
+    def __init__(self, specs, seed, coefficients=None):
         self.specs = specs
         self.keep = []
         self.blocks = (wm.Block * len(specs))()
@@ -46,7 +46,8 @@ class Blocks:
             cs3, fc = [], []
             for d in range(3):
                 ns, nk, nl = fields.dir_extents(m, n, p, 6, d)
-                cs = fields.random_coefficients(ns - 6, seed=seed * 7 + b * 3 + d)
+                cs = (fields.random_coefficients(ns - 6, seed=seed * 7 + b * 3 + d) if coefficients is None
+                      else coefficients(ns - 6, order, seed * 7 + b * 3 + d))
                 devs = [DeviceArray.from_host(c) for c in cs]
                 blk.coeffs[d] = wm.Coeffs(*[x.ptr for x in devs])
                 L, R = DeviceArray((ns, nk, nl, v)), DeviceArray((ns, nk, nl, v))

@@ -1,0 +1,162 @@
+"""The reference's other weight branches on the HIP path: mapped WENO, WENO-Z
+and TENO (compile-time parameters mapped_weno, wenoz, teno of
+inputs/weno_standalone.f90:72-74, runtime switches of weno_grid here).
+
+Tolerances:
+  FP64 ........ 1e-13 relative (rel_err) against the reference's own outputs
+                (tests/golden/ref_branches.npz, written by the reference built
+                with the parameter set).  Against the oracle on non-cubic boxes
+                with j-varying coefficients (smoothness indicators positive
+                semi-definite, fields.psd_coefficients; candidate coefficients
+                up to 0.6 across a 1e3 jump, where WENO-Z's weaker damping
+                lets the candidates cancel): 1e-13 relative to max(|face|,
+                largest |v| of the face's stencil) (rel_err_stencil)
+  FP32 state .. 1e-6 relative to the FP64 oracle on the FP32-rounded field,
+                same scale
+TENO is checked on the harness coefficients only: its cut-off is
+discontinuous, and these inputs keep every weight far from teno_CT."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import wenostandalone_amd as wm
+from wenostandalone_amd import fields
+from wenostandalone_amd.device import DeviceArray
+from tests.conftest import GOLDEN, rel_err, rel_err_stencil, stencil_scale
+from tests.test_gpu_batched import SPECS, Blocks
+
+pytestmark = pytest.mark.gpu
+
+BRANCH_KW = {"teno": {"teno": True}, "wenoz": {"wenoz": True}, "mapped_weno": {"mapped_weno": True}}
+ORDERS = {"weno3": 3, "weno5": 5, "weno7": 7}
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _init():
+    wm.check(wm.lib.weno_init(0))
+    yield
+
+
+def _coef(cs):
+    devs = [DeviceArray.from_host(c, np.float64) for c in cs]
+    return devs, wm.Coeffs(*[d.ptr for d in devs])
+
+
+def _run(g, order, d, src, kind, coef, flags, shape, pre=None, dtype=wm.F64):
+    odt = np.float64 if dtype == wm.F64 else np.float32
+    vL, vR = DeviceArray(shape, odt), DeviceArray(shape, odt)
+    if pre is None:
+        vL.fill(7.25)
+        vR.fill(-3.5)
+    else:
+        vL.upload(pre[0])
+        vR.upload(pre[1])
+    wm.check(wm.lib.weno_reconstruct(C.byref(g), order, d, kind, dtype, src.vp, C.byref(coef), vL.vp, vR.vp,
+                                     flags))
+    return vL.download(), vR.download()
+
+
+def test_branch_fixtures():
+    """The reference's own TENO / WENO-Z / mapped outputs (16^3 box, recipe-2
+    field with a 1e3 jump and exactly constant boxes), x, y and z."""
+    ref = np.load(os.path.join(GOLDEN, "ref_branches.npz"))
+    m, v, b = int(ref["m"]), int(ref["v"]), int(ref["buff"])
+    vf = DeviceArray.from_host(ref["field"])
+    n = 0
+    for branch, kw in BRANCH_KW.items():
+        g = wm.make_grid(m, v=v, buff=b, wenojs=False, **kw)
+        for fam, order in ORDERS.items():
+            if f"{branch}_{fam}_x_L" not in ref.files:
+                continue
+            for d, name in enumerate("xyz"):
+                ns, nk, nl = fields.dir_extents(m, m, m, b, d)
+                devs, coef = _coef(fields.harness_coefficients(ns - 6, fields.DIR_OFFSETS[d]))
+                L, R = _run(g, order, d, vf, wm.SRC_FIELD, coef, wm.FUSE_COPY, (ns, nk, nl, v))
+                assert rel_err(L[3:-3], ref[f"{branch}_{fam}_{name}_L"]) <= 1e-13, (branch, fam, name)
+                assert rel_err(R[3:-3], ref[f"{branch}_{fam}_{name}_R"]) <= 1e-13, (branch, fam, name)
+                n += 1
+    assert n == 24
+
+
+@pytest.mark.parametrize("branch", ["wenoz", "mapped_weno"])
+@pytest.mark.parametrize("order", [3, 5, 7])
+def test_branch_vs_oracle_noncubic(branch, order):
+    """Non-cubic box, j-varying coefficients, both source layouts, plain /
+    fused-copy / fused-MP launches, FP64 and FP32 state."""
+    m, n, p, v = 9, 12, 7, 3
+    kw = BRANCH_KW[branch]
+    g = wm.make_grid(m, n, p, v=v, buff=6, wenojs=False, **kw)
+    og = oracle.grid(m, n, p, v=v, buff=6, wenojs=False, **kw)
+    f = fields.seeded_field(m, n, p, v, 6, seed=600 + order, recipe=2)
+    for d in range(3):
+        ns, nk, nl = fields.dir_extents(m, n, p, 6, d)
+        cs = fields.psd_coefficients(ns - 6, order, seed=900 + 10 * order + d)
+        devs, coef = _coef(cs)
+        v_rs = oracle.reshape(og, d, f)
+        sc = stencil_scale(v_rs, order // 2 if order != 3 else 1)
+        cL, cR = oracle.order1_copy(og, d, f)
+        eL, eR = oracle.weno(og, order, d, v_rs, cs, cL, cR)
+        mL, mR = oracle.mp(og, d, v_rs, eL, eR)
+        vf, rs = DeviceArray.from_host(f), DeviceArray.from_host(v_rs)
+        shape = (ns, nk, nl, v)
+        for src, kind in [(vf, wm.SRC_FIELD), (rs, wm.SRC_RESHAPED)]:
+            L, R = _run(g, order, d, src, kind, coef, 0, shape, pre=(cL, cR))
+            assert rel_err_stencil(L, eL, sc) <= 1e-13 and rel_err_stencil(R, eR, sc) <= 1e-13, (d, kind)
+            L, R = _run(g, order, d, src, kind, coef, wm.FUSE_COPY | wm.FUSE_MP, shape)
+            assert rel_err_stencil(L, mL, sc) <= 1e-13 and rel_err_stencil(R, mR, sc) <= 1e-13, (d, kind)
+        f32 = f.astype(np.float32)
+        f64r = f32.astype(np.float64)
+        cL, cR = oracle.order1_copy(og, d, f64r)
+        eL, eR = oracle.weno(og, order, d, oracle.reshape(og, d, f64r), cs, cL, cR)
+        L, R = _run(g, order, d, DeviceArray.from_host(f32), wm.SRC_FIELD, coef, wm.FUSE_COPY, shape,
+                    dtype=wm.F32)
+        assert rel_err_stencil(L, eL, sc) <= 1e-6 and rel_err_stencil(R, eR, sc) <= 1e-6, d
+
+
+def test_branch_precedence():
+    """wenojs wins over the other switches; mapped_weno over wenoz and teno;
+    WENO3 with teno alone is NaN (the reference has no TENO3 branch)."""
+    m, v = 8, 2
+    f = fields.seeded_field(m, m, m, v, 6, seed=77)
+    ns, nk, nl = fields.dir_extents(m, m, m, 6, 0)
+    devs, coef = _coef(fields.harness_coefficients(ns - 6, 0.03))
+    vf = DeviceArray.from_host(f)
+    shape = (ns, nk, nl, v)
+    for order in (3, 5, 7):
+        js = _run(wm.make_grid(m, v=v), order, 0, vf, wm.SRC_FIELD, coef, wm.FUSE_COPY, shape)
+        every = _run(wm.make_grid(m, v=v, mapped_weno=True, wenoz=True, teno=True), order, 0, vf, wm.SRC_FIELD,
+                     coef, wm.FUSE_COPY, shape)
+        assert all(np.array_equal(a, b) for a, b in zip(js, every))
+        mp = _run(wm.make_grid(m, v=v, wenojs=False, mapped_weno=True), order, 0, vf, wm.SRC_FIELD, coef,
+                  wm.FUSE_COPY, shape)
+        mpz = _run(wm.make_grid(m, v=v, wenojs=False, mapped_weno=True, wenoz=True, teno=True), order, 0, vf,
+                   wm.SRC_FIELD, coef, wm.FUSE_COPY, shape)
+        assert all(np.array_equal(a, b) for a, b in zip(mp, mpz))
+    L, R = _run(wm.make_grid(m, v=v, wenojs=False, teno=True), 3, 0, vf, wm.SRC_FIELD, coef, wm.FUSE_COPY, shape)
+    assert np.isnan(L).all() and np.isnan(R).all()
+
+
+@pytest.mark.parametrize("branch", ["wenoz", "mapped_weno"])
+def test_batched_branch(branch):
+    """Mixed WENO3/5/7 blocks of one weight branch in the batched launch."""
+    kw = BRANCH_KW[branch]
+    bl = Blocks(SPECS, seed=700, coefficients=fields.psd_coefficients)
+    for b, (m, n, p, v, order) in enumerate(SPECS):
+        bl.blocks[b].grid = wm.make_grid(m, n, p, v=v, buff=6, wenojs=False, **kw)
+    bl.fill(9.5)
+    for d in range(3):
+        wm.check(wm.lib.weno_batched_reconstruct(bl.blocks, len(SPECS), d, wm.FUSE_COPY))
+    got = bl.download()
+    for b, (m, n, p, v, order) in enumerate(SPECS):
+        og = oracle.grid(m, n, p, v=v, buff=6, wenojs=False, **kw)
+        for d in range(3):
+            f = bl.fields[b]
+            v_rs = oracle.reshape(og, d, f)
+            cL, cR = oracle.order1_copy(og, d, f)
+            eL, eR = oracle.weno(og, order, d, v_rs, bl.coefs[b][d], cL, cR)
+            sc = stencil_scale(v_rs, order // 2 if order != 3 else 1)
+            L, R = got[b][d]
+            assert rel_err_stencil(L, eL, sc) <= 1e-13 and rel_err_stencil(R, eR, sc) <= 1e-13, (b, d)

@@ -1,8 +1,9 @@
 """Pins the CPU oracle (oracle/weno_oracle.c) before it is trusted as the
 checker: against the reference's outputs (fixtures written by the reference's
 own kernels, the reference's golden checksums, and -- in this container -- the
-live reference build), and against closed forms for what no reference output
-covers (TENO, the WENO7-TENO quirk, MP invariants, the VJP)."""
+live reference build, also built with its TENO / WENO-Z / mapped-WENO
+parameters set), and against closed forms (TENO, the WENO7-TENO quirk, MP
+invariants, the VJP, which has no reference counterpart)."""
 import json
 import os
 from fractions import Fraction as Fr
@@ -93,6 +94,71 @@ def test_live_reference_random_fields():
                 L, R = oracle.sequence(g, fam, d, f, harness_coefs(g, d))
                 s = slice(3, -3) if fam == "weno3" else slice(None)
                 assert np.array_equal(L[s], ref[2 * d][s]) and np.array_equal(R[s], ref[2 * d + 1][s])
+
+
+BRANCH_KW = {"teno": {"teno": True}, "wenoz": {"wenoz": True}, "mapped_weno": {"mapped_weno": True}}
+
+
+def test_branch_fixtures_bitwise():
+    """TENO, WENO-Z and mapped WENO reproduced bit for bit against the
+    reference built with the matching compile-time parameter set
+    (tests/golden/ref_branches.npz, core range)."""
+    ref = np.load(os.path.join(GOLDEN, "ref_branches.npz"))
+    m, v, b = int(ref["m"]), int(ref["v"]), int(ref["buff"])
+    f = ref["field"]
+    n = 0
+    for branch, kw in BRANCH_KW.items():
+        g = oracle.grid(m, v=v, buff=b, wenojs=False, **kw)
+        for fam in ["weno3", "weno5", "weno7"]:
+            if f"{branch}_{fam}_x_L" not in ref.files:
+                continue
+            for d, name in enumerate("xyz"):
+                L, R = oracle.sequence(g, fam, d, f, harness_coefs(g, d))
+                assert np.array_equal(L[3:-3], ref[f"{branch}_{fam}_{name}_L"]), (branch, fam, name)
+                assert np.array_equal(R[3:-3], ref[f"{branch}_{fam}_{name}_R"]), (branch, fam, name)
+                n += 1
+    assert n == 3 * 8
+
+
+def test_weight_branch_precedence():
+    """wenojs wins over every other branch (:552), mapped_weno over wenoz and
+    teno, wenoz over teno (:557, :572, :578); WENO3 has no TENO branch, so
+    teno alone leaves it NaN."""
+    m, v, b = 3, 1, 6
+    f = fields.seeded_field(m, m, m, v, b, seed=4, recipe=2)
+    def run(order, **kw):
+        g = oracle.grid(m, v=v, buff=b, **kw)
+        return oracle.sequence(g, {3: "weno3", 5: "weno5", 7: "weno7"}[order], 0, f, harness_coefs(g, 0))
+    for order in (3, 5, 7):
+        js = run(order)
+        allon = run(order, wenojs=True, mapped_weno=True, wenoz=True, teno=True)
+        assert all(np.array_equal(a, c) for a, c in zip(js, allon))
+        mp = run(order, wenojs=False, mapped_weno=True)
+        assert all(np.array_equal(a, c) for a, c in zip(mp, run(order, wenojs=False, mapped_weno=True, wenoz=True,
+                                                              teno=True)))
+        z = run(order, wenojs=False, wenoz=True)
+        assert all(np.array_equal(a, c) for a, c in zip(z, run(order, wenojs=False, wenoz=True, teno=True)))
+        assert not np.array_equal(js[0][3:-3], z[0][3:-3]) and not np.array_equal(js[0][3:-3], mp[0][3:-3])
+    L, R = run(3, wenojs=False, teno=True)
+    assert np.isnan(L[3:-3]).all() and np.isnan(R[3:-3]).all()
+
+
+@pytest.mark.skipif(not all(os.path.exists(oracle.ref_driver(b)) for b in oracle.BRANCHES),
+                    reason="reference branch builds absent (only in the build container)")
+def test_live_reference_branches():
+    """Fresh seeded fields through the live reference branch builds."""
+    for seed, recipe in [(303, 0), (404, 2)]:
+        m, v, b = 4, 2, 6
+        f = fields.seeded_field(m, m, m, v, b, seed=seed, recipe=recipe)
+        for branch, kw in BRANCH_KW.items():
+            g = oracle.grid(m, v=v, buff=b, wenojs=False, **kw)
+            for fam in ["weno3", "weno5", "weno7", "mp5", "mp7"]:
+                ref = oracle.run_ref(fam, m, v, b, f, wenojs=False, branch=branch)
+                for d in range(3):
+                    L, R = oracle.sequence(g, fam, d, f, harness_coefs(g, d))
+                    s = slice(3, -3) if fam == "weno3" else slice(None)
+                    assert np.array_equal(L[s], ref[2 * d][s], equal_nan=True), (branch, fam, d)
+                    assert np.array_equal(R[s], ref[2 * d + 1][s], equal_nan=True), (branch, fam, d)
 
 
 # ---------------------------------------------------------------------------

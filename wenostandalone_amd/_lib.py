@@ -28,13 +28,15 @@ class WenoError(RuntimeError):
 
 
 class Grid(C.Structure):
-    """weno_grid: namelist &grid_params (weno_standalone.f90:57-66) + teno."""
+    """weno_grid: namelist &grid_params (weno_standalone.f90:57-66) + the
+    weight-branch switches teno, mapped_weno, wenoz (compile-time parameters
+    of the reference, :72-74)."""
 
     _fields_ = [
         ("m_cells", C.c_int), ("n_cells", C.c_int), ("p_cells", C.c_int),
         ("v_size", C.c_int), ("buff_size", C.c_int),
         ("weno_eps", C.c_double), ("wenojs", C.c_int), ("teno", C.c_int),
-        ("teno_CT", C.c_double),
+        ("teno_CT", C.c_double), ("mapped_weno", C.c_int), ("wenoz", C.c_int),
     ]
 
 
@@ -126,7 +128,8 @@ def check(rc: int) -> int:
 
 def make_grid(m: int, n: int | None = None, p: int | None = None, v: int = 8, buff: int = 6,
               eps: float = 1e-16, wenojs: bool = True, teno: bool = False,
-              teno_ct: float = 1e-5) -> Grid:
+              teno_ct: float = 1e-5, mapped_weno: bool = False, wenoz: bool = False) -> Grid:
     n = m if n is None else n
     p = m if p is None else p
-    return Grid(m, n, p, v, buff, eps, int(bool(wenojs)), int(bool(teno)), teno_ct)
+    return Grid(m, n, p, v, buff, eps, int(bool(wenojs)), int(bool(teno)), teno_ct,
+                int(bool(mapped_weno)), int(bool(wenoz)))

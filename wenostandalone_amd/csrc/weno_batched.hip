@@ -204,13 +204,19 @@ __global__ void __launch_bounds__(kBatchThreads)
         if (order == 5) {
             if (mode == MODE_JS) WM_BATCH(5, MODE_JS);
             else if (mode == MODE_TENO) WM_BATCH(5, MODE_TENO);
+            else if (mode == MODE_MAPPED) WM_BATCH(5, MODE_MAPPED);
+            else if (mode == MODE_WENOZ) WM_BATCH(5, MODE_WENOZ);
             else WM_BATCH(5, MODE_NONE);
         } else if (order == 7) {
             if (mode == MODE_JS) WM_BATCH(7, MODE_JS);
             else if (mode == MODE_TENO) WM_BATCH(7, MODE_TENO);
+            else if (mode == MODE_MAPPED) WM_BATCH(7, MODE_MAPPED);
+            else if (mode == MODE_WENOZ) WM_BATCH(7, MODE_WENOZ);
             else WM_BATCH(7, MODE_NONE);
         } else {
             if (mode == MODE_JS) WM_BATCH(3, MODE_JS);
+            else if (mode == MODE_MAPPED) WM_BATCH(3, MODE_MAPPED);
+            else if (mode == MODE_WENOZ) WM_BATCH(3, MODE_WENOZ);
             else WM_BATCH(3, MODE_NONE);
         }
 #undef WM_BATCH

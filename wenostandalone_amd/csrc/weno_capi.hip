@@ -257,7 +257,10 @@ void plan_tiles(TileArgs& a, Launch& l, int np_override, int chunk_override, int
 
 int mode_for(const weno_grid* g, int order)
 {
+    // precedence of the reference's if/elseif chain (:552, :557, :572, :578)
     if (g->wenojs) return MODE_JS;
+    if (g->mapped_weno) return MODE_MAPPED;
+    if (g->wenoz) return MODE_WENOZ;
     if (g->teno && order != 3) return MODE_TENO;
     return MODE_NONE;
 }

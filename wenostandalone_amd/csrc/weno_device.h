@@ -232,27 +232,172 @@ __device__ __forceinline__ double masked_combine(const double (&a)[N], const C (
     return num / (S + extra);
 }
 
+// Mapped WENO (weno_standalone.f90:433-437, :557-570, :1478-1482): the JS
+// weights w_r = a_r / sum(a), a_r = d_r / beta_r^2, mapped through
+//   g_r(w) = (d_r (1 + d_r - 3 w) + w^2) * (w / (d_r^2 + w (1 - 2 d_r)))
+// (Henrick, Aslam & Powers 2005), then normalised again.  IEEE divisions, as
+// the reference: the mapping is evaluated once per face and side.
+template <int N, typename C>
+__device__ __forceinline__ double mapped_combine(const double (&q)[N], const double (&d)[N], const C (&p)[N])
+{
+    double a[N], s = 0.0;
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+        a[r] = d[r] / q[r];
+        s += a[r];
+    }
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+        const double w = a[r] / s;
+        a[r] = (d[r] * (1.0 + d[r] - 3.0 * w) + w * w) * (w / (d[r] * d[r] + w * (1.0 - 2.0 * d[r])));
+    }
+    return masked_combine<N, C>(a, p, 0.0);
+}
+
+// WENO-Z (weno_standalone.f90:439-441, :572-576, :1484-1488):
+// a_r = d_r (1 + (tau / beta_r)^Q), Q = 1 for WENO3/5 and wenoz_q = 2 for
+// WENO7 as the reference writes them.
+template <int N, int Q, typename C>
+__device__ __forceinline__ double wenoz_combine(const double (&b)[N], double tau, const double (&d)[N],
+                                                const C (&p)[N])
+{
+    double a[N];
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+        const double t = tau / b[r];
+        a[r] = d[r] * (1.0 + (Q == 2 ? t * t : t));
+    }
+    return masked_combine<N, C>(a, p, 0.0);
+}
+
+// ---- stencil terms: candidate polynomials and smoothness indicators --------
+// Each body is written in the reference's evaluation order (left-to-right
+// sums and products, :421-428, :529-550, :1450-1473) and instantiated twice:
+// with contraction (FMA) for the JS weights, which are well conditioned in
+// beta, and without (`fp contract(off)`: HIP's default contraction mode
+// honours the pragma) for the mapped, WENO-Z and TENO branches.  Their
+// weights take differences of betas (tau = |beta_a - beta_b|) or cut at a
+// threshold, so one-ulp differences in beta would be amplified; evaluated
+// exactly as the reference does, beta and tau are bit-identical to it.
+#define WM_TERMS3                                                                           \
+    const C d0 = v[2] - v[1], dm1 = v[1] - v[0];                                            \
+    pL[0] = v[1] + f.pl[0] * d0;                                                            \
+    pL[1] = v[1] + f.pl[1] * dm1;                                                           \
+    pR[0] = v[1] + f.pr[0] * d0;                                                            \
+    pR[1] = v[1] + f.pr[1] * dm1;                                                           \
+    const double D0 = d0, Dm1 = dm1;                                                        \
+    b0 = B(0, 0) * D0 * D0 + eps;                                                           \
+    b1 = B(1, 0) * Dm1 * Dm1 + eps;
+#define WM_TERMS5                                                                           \
+    const C vj = v[2];                                                                      \
+    const C dp1 = v[4] - v[3], d0 = v[3] - v[2], dm1 = v[2] - v[1], dm2 = v[1] - v[0];      \
+    pL[0] = vj + f.pl[0][0] * dp1 + f.pl[0][1] * d0;                                        \
+    pL[1] = vj + f.pl[1][0] * d0 + f.pl[1][1] * dm1;                                        \
+    pL[2] = vj + f.pl[2][0] * dm1 + f.pl[2][1] * dm2;                                       \
+    pR[0] = vj + f.pr[0][0] * dp1 + f.pr[0][1] * d0;                                        \
+    pR[1] = vj + f.pr[1][0] * d0 + f.pr[1][1] * dm1;                                        \
+    pR[2] = vj + f.pr[2][0] * dm1 + f.pr[2][1] * dm2;                                       \
+    const double Dp1 = dp1, D0 = d0, Dm1 = dm1, Dm2 = dm2;                                  \
+    b0 = B(0, 0) * Dp1 * Dp1 + B(0, 1) * Dp1 * D0 + B(0, 2) * D0 * D0 + eps;                \
+    b1 = B(1, 0) * D0 * D0 + B(1, 1) * D0 * Dm1 + B(1, 2) * Dm1 * Dm1 + eps;                \
+    b2 = B(2, 0) * Dm1 * Dm1 + B(2, 1) * Dm1 * Dm2 + B(2, 2) * Dm2 * Dm2 + eps;
+// WENO7: poly(q)/beta(q) use coefficient row 3-q (:1457-1473)
+#define WM_TERMS7                                                                           \
+    const C vj = v[3];                                                                      \
+    const C d2 = v[6] - v[5], d1 = v[5] - v[4], d0 = v[4] - v[3];                           \
+    const C dm1 = v[3] - v[2], dm2 = v[2] - v[1], dm3 = v[1] - v[0];                        \
+    pL[0] = vj + f.pl[3][0] * dm1 + f.pl[3][1] * dm2 + f.pl[3][2] * dm3;                    \
+    pL[1] = vj + f.pl[2][0] * d0 + f.pl[2][1] * dm1 + f.pl[2][2] * dm2;                     \
+    pL[2] = vj + f.pl[1][0] * d1 + f.pl[1][1] * d0 + f.pl[1][2] * dm1;                      \
+    pL[3] = vj + f.pl[0][0] * d2 + f.pl[0][1] * d1 + f.pl[0][2] * d0;                       \
+    pR[0] = vj + f.pr[3][0] * dm1 + f.pr[3][1] * dm2 + f.pr[3][2] * dm3;                    \
+    pR[1] = vj + f.pr[2][0] * d0 + f.pr[2][1] * dm1 + f.pr[2][2] * dm2;                     \
+    pR[2] = vj + f.pr[1][0] * d1 + f.pr[1][1] * d0 + f.pr[1][2] * dm1;                      \
+    pR[3] = vj + f.pr[0][0] * d2 + f.pr[0][1] * d1 + f.pr[0][2] * d0;                       \
+    const double D2 = d2, D1 = d1, D0 = d0, Dm1 = dm1, Dm2 = dm2, Dm3 = dm3;                \
+    b[3] = B(0, 0) * D2 * D2 + B(0, 1) * D2 * D1 + B(0, 2) * D2 * D0 + B(0, 3) * D1 * D1 +  \
+           B(0, 4) * D1 * D0 + B(0, 5) * D0 * D0 + eps;                                     \
+    b[2] = B(1, 0) * D1 * D1 + B(1, 1) * D1 * D0 + B(1, 2) * D1 * Dm1 + B(1, 3) * D0 * D0 + \
+           B(1, 4) * D0 * Dm1 + B(1, 5) * Dm1 * Dm1 + eps;                                  \
+    b[1] = B(2, 0) * D0 * D0 + B(2, 1) * D0 * Dm1 + B(2, 2) * D0 * Dm2 + B(2, 3) * Dm1 * Dm1 + \
+           B(2, 4) * Dm1 * Dm2 + B(2, 5) * Dm2 * Dm2 + eps;                                 \
+    b[0] = B(3, 0) * Dm1 * Dm1 + B(3, 1) * Dm1 * Dm2 + B(3, 2) * Dm1 * Dm3 + B(3, 3) * Dm2 * Dm2 + \
+           B(3, 4) * Dm2 * Dm3 + B(3, 5) * Dm3 * Dm3 + eps;
+
+#define WM_TERMS_FNS(O, ARGS, BODY)                                                         \
+    template <typename C, typename BT>                                                      \
+    __device__ __forceinline__ void terms##O##_fused(const C* v, const Cf<O, C>& f, const BT& B, double eps, ARGS) \
+    {                                                                                       \
+        BODY                                                                                \
+    }                                                                                       \
+    template <typename C, typename BT>                                                      \
+    __device__ __forceinline__ void terms##O##_exact(const C* v, const Cf<O, C>& f, const BT& B, double eps, ARGS) \
+    {                                                                                       \
+        _Pragma("clang fp contract(off)")                                                   \
+        BODY                                                                                \
+    }
+#define WM_COMMA ,
+WM_TERMS_FNS(3, C (&pL)[2] WM_COMMA C (&pR)[2] WM_COMMA double& b0 WM_COMMA double& b1, WM_TERMS3)
+WM_TERMS_FNS(5, C (&pL)[3] WM_COMMA C (&pR)[3] WM_COMMA double& b0 WM_COMMA double& b1 WM_COMMA double& b2,
+             WM_TERMS5)
+WM_TERMS_FNS(7, C (&pL)[4] WM_COMMA C (&pR)[4] WM_COMMA double (&b)[4], WM_TERMS7)
+#undef WM_TERMS_FNS
+#undef WM_COMMA
+#undef WM_TERMS3
+#undef WM_TERMS5
+#undef WM_TERMS7
+
+template <bool EXACT, typename C, typename BT>
+__device__ __forceinline__ void terms3(const C* v, const Cf<3, C>& f, const BT& B, double eps, C (&pL)[2],
+                                       C (&pR)[2], double& b0, double& b1)
+{
+    if constexpr (EXACT) terms3_exact(v, f, B, eps, pL, pR, b0, b1);
+    else terms3_fused(v, f, B, eps, pL, pR, b0, b1);
+}
+template <bool EXACT, typename C, typename BT>
+__device__ __forceinline__ void terms5(const C* v, const Cf<5, C>& f, const BT& B, double eps, C (&pL)[3],
+                                       C (&pR)[3], double& b0, double& b1, double& b2)
+{
+    if constexpr (EXACT) terms5_exact(v, f, B, eps, pL, pR, b0, b1, b2);
+    else terms5_fused(v, f, B, eps, pL, pR, b0, b1, b2);
+}
+template <bool EXACT, typename C, typename BT>
+__device__ __forceinline__ void terms7(const C* v, const Cf<7, C>& f, const BT& B, double eps, C (&pL)[4],
+                                       C (&pR)[4], double (&b)[4])
+{
+    if constexpr (EXACT) terms7_exact(v, f, B, eps, pL, pR, b);
+    else terms7_fused(v, f, B, eps, pL, pR, b);
+}
+
 // ---- WENO3: v[0..2] = j-1, j, j+1 (weno_standalone.f90:392-475) ----------
 template <int MODE, typename C, typename BT>
 __device__ __forceinline__ void weno3(const C* v, const Cf<3, C>& f, const BT& B, double eps,
                                       double, double& oL, double& oR)
 {
-    if constexpr (MODE != MODE_JS) {
+    // no TENO branch for WENO3 in the reference: teno alone leaves NaN
+    if constexpr (MODE == MODE_NONE || MODE == MODE_TENO) {
         oL = oR = __builtin_nan("");
         return;
     } else {
-        const C d0 = v[2] - v[1], dm1 = v[1] - v[0];
-        const C pL[2] = {v[1] + f.pl[0] * d0, v[1] + f.pl[1] * dm1};
-        const C pR[2] = {v[1] + f.pr[0] * d0, v[1] + f.pr[1] * dm1};
-        const double D0 = d0, Dm1 = dm1;
-        const double b0 = B(0, 0) * D0 * D0 + eps;
-        const double b1 = B(1, 0) * Dm1 * Dm1 + eps;
+        C pL[2], pR[2];
+        double b0, b1;
+        terms3<MODE != MODE_JS>(v, f, B, eps, pL, pR, b0, b1);
         const double q[2] = {b0 * b0, b1 * b1};
-        double P[2];
-        excl_products<2>(q, P);
         const double dl[2] = {B.dl(0), B.dl(1)}, dr[2] = {B.dr(0), B.dr(1)};
-        oL = js_combine<2, C>(q, P, dl, pL);
-        oR = js_combine<2, C>(q, P, dr, pR);
+        if constexpr (MODE == MODE_JS) {
+            double P[2];
+            excl_products<2>(q, P);
+            oL = js_combine<2, C>(q, P, dl, pL);
+            oR = js_combine<2, C>(q, P, dr, pR);
+        } else if constexpr (MODE == MODE_MAPPED) {
+            oL = mapped_combine<2, C>(q, dl, pL);
+            oR = mapped_combine<2, C>(q, dr, pR);
+        } else {
+            const double bb[2] = {b0, b1};
+            const double tau = fabs(b1 - b0);
+            oL = wenoz_combine<2, 1, C>(bb, tau, dl, pL);
+            oR = wenoz_combine<2, 1, C>(bb, tau, dr, pR);
+        }
     }
 }
 
@@ -265,18 +410,9 @@ __device__ __forceinline__ void weno5(const C* v, const Cf<5, C>& f, const BT& B
         oL = oR = __builtin_nan("");
         return;
     } else {
-        const C vj = v[2];
-        const C dp1 = v[4] - v[3], d0 = v[3] - v[2], dm1 = v[2] - v[1], dm2 = v[1] - v[0];
-        const C pL[3] = {vj + f.pl[0][0] * dp1 + f.pl[0][1] * d0,
-                         vj + f.pl[1][0] * d0 + f.pl[1][1] * dm1,
-                         vj + f.pl[2][0] * dm1 + f.pl[2][1] * dm2};
-        const C pR[3] = {vj + f.pr[0][0] * dp1 + f.pr[0][1] * d0,
-                         vj + f.pr[1][0] * d0 + f.pr[1][1] * dm1,
-                         vj + f.pr[2][0] * dm1 + f.pr[2][1] * dm2};
-        const double Dp1 = dp1, D0 = d0, Dm1 = dm1, Dm2 = dm2;
-        const double b0 = B(0, 0) * Dp1 * Dp1 + B(0, 1) * Dp1 * D0 + B(0, 2) * D0 * D0 + eps;
-        const double b1 = B(1, 0) * D0 * D0 + B(1, 1) * D0 * Dm1 + B(1, 2) * Dm1 * Dm1 + eps;
-        const double b2 = B(2, 0) * Dm1 * Dm1 + B(2, 1) * Dm1 * Dm2 + B(2, 2) * Dm2 * Dm2 + eps;
+        C pL[3], pR[3];
+        double b0, b1, b2;
+        terms5<MODE != MODE_JS>(v, f, B, eps, pL, pR, b0, b1, b2);
         if constexpr (MODE == MODE_JS) {
             const double q[3] = {b0 * b0, b1 * b1, b2 * b2};
             double P[3];
@@ -284,6 +420,17 @@ __device__ __forceinline__ void weno5(const C* v, const Cf<5, C>& f, const BT& B
             const double dl[3] = {B.dl(0), B.dl(1), B.dl(2)}, dr[3] = {B.dr(0), B.dr(1), B.dr(2)};
             oL = js_combine<3, C>(q, P, dl, pL);
             oR = js_combine<3, C>(q, P, dr, pR);
+        } else if constexpr (MODE == MODE_MAPPED) {
+            const double q[3] = {b0 * b0, b1 * b1, b2 * b2};
+            const double dl[3] = {B.dl(0), B.dl(1), B.dl(2)}, dr[3] = {B.dr(0), B.dr(1), B.dr(2)};
+            oL = mapped_combine<3, C>(q, dl, pL);
+            oR = mapped_combine<3, C>(q, dr, pR);
+        } else if constexpr (MODE == MODE_WENOZ) {
+            const double bb[3] = {b0, b1, b2};
+            const double tau = fabs(b2 - b0);
+            const double dl[3] = {B.dl(0), B.dl(1), B.dl(2)}, dr[3] = {B.dr(0), B.dr(1), B.dr(2)};
+            oL = wenoz_combine<3, 1, C>(bb, tau, dl, pL);
+            oR = wenoz_combine<3, 1, C>(bb, tau, dr, pR);
         } else {
             // :578-606, kept with IEEE divisions and pow
             const double tau = fabs(b2 - b0);
@@ -313,39 +460,9 @@ __device__ __forceinline__ void weno7(const C* v, const Cf<7, C>& f, const BT& B
         oL = oR = __builtin_nan("");
         return;
     } else {
-        const C vj = v[3];
-        const C d2 = v[6] - v[5], d1 = v[5] - v[4], d0 = v[4] - v[3];
-        const C dm1 = v[3] - v[2], dm2 = v[2] - v[1], dm3 = v[1] - v[0];
-        const C pL[4] = {vj + f.pl[3][0] * dm1 + f.pl[3][1] * dm2 + f.pl[3][2] * dm3,
-                         vj + f.pl[2][0] * d0 + f.pl[2][1] * dm1 + f.pl[2][2] * dm2,
-                         vj + f.pl[1][0] * d1 + f.pl[1][1] * d0 + f.pl[1][2] * dm1,
-                         vj + f.pl[0][0] * d2 + f.pl[0][1] * d1 + f.pl[0][2] * d0};
-        const C pR[4] = {vj + f.pr[3][0] * dm1 + f.pr[3][1] * dm2 + f.pr[3][2] * dm3,
-                         vj + f.pr[2][0] * d0 + f.pr[2][1] * dm1 + f.pr[2][2] * dm2,
-                         vj + f.pr[1][0] * d1 + f.pr[1][1] * d0 + f.pr[1][2] * dm1,
-                         vj + f.pr[0][0] * d2 + f.pr[0][1] * d1 + f.pr[0][2] * d0};
-        const double D2 = d2, D1 = d1, D0 = d0, Dm1 = dm1, Dm2 = dm2, Dm3 = dm3;
+        C pL[4], pR[4];
         double b[4];
-        {
-            auto c = [&](int t) { return B(0, t); };
-            b[3] = c(0) * D2 * D2 + c(1) * D2 * D1 + c(2) * D2 * D0 + c(3) * D1 * D1 + c(4) * D1 * D0 +
-                   c(5) * D0 * D0 + eps;
-        }
-        {
-            auto c = [&](int t) { return B(1, t); };
-            b[2] = c(0) * D1 * D1 + c(1) * D1 * D0 + c(2) * D1 * Dm1 + c(3) * D0 * D0 + c(4) * D0 * Dm1 +
-                   c(5) * Dm1 * Dm1 + eps;
-        }
-        {
-            auto c = [&](int t) { return B(2, t); };
-            b[1] = c(0) * D0 * D0 + c(1) * D0 * Dm1 + c(2) * D0 * Dm2 + c(3) * Dm1 * Dm1 +
-                   c(4) * Dm1 * Dm2 + c(5) * Dm2 * Dm2 + eps;
-        }
-        {
-            auto c = [&](int t) { return B(3, t); };
-            b[0] = c(0) * Dm1 * Dm1 + c(1) * Dm1 * Dm2 + c(2) * Dm1 * Dm3 + c(3) * Dm2 * Dm2 +
-                   c(4) * Dm2 * Dm3 + c(5) * Dm3 * Dm3 + eps;
-        }
+        terms7<MODE != MODE_JS>(v, f, B, eps, pL, pR, b);
         if constexpr (MODE == MODE_JS) {
             const double q[4] = {b[0] * b[0], b[1] * b[1], b[2] * b[2], b[3] * b[3]};
             double P[4];
@@ -354,17 +471,34 @@ __device__ __forceinline__ void weno7(const C* v, const Cf<7, C>& f, const BT& B
             const double dr[4] = {B.dr(0), B.dr(1), B.dr(2), B.dr(3)};
             oL = js_combine<4, C>(q, P, dl, pL);
             oR = js_combine<4, C>(q, P, dr, pR);
+        } else if constexpr (MODE == MODE_MAPPED) {
+            // alpha(4) = 0 joins the sums of :1480 and changes nothing
+            const double q[4] = {b[0] * b[0], b[1] * b[1], b[2] * b[2], b[3] * b[3]};
+            const double dl[4] = {B.dl(0), B.dl(1), B.dl(2), B.dl(3)};
+            const double dr[4] = {B.dr(0), B.dr(1), B.dr(2), B.dr(3)};
+            oL = mapped_combine<4, C>(q, dl, pL);
+            oR = mapped_combine<4, C>(q, dr, pR);
+        } else if constexpr (MODE == MODE_WENOZ) {
+            const double tau = fabs(b[3] - b[0]);
+            const double dl[4] = {B.dl(0), B.dl(1), B.dl(2), B.dl(3)};
+            const double dr[4] = {B.dr(0), B.dr(1), B.dr(2), B.dr(3)};
+            oL = wenoz_combine<4, 2, C>(b, tau, dl, pL);
+            oR = wenoz_combine<4, 2, C>(b, tau, dr, pR);
         } else {
-            // :1490-1503: beta(4) keeps weno_eps, alpha(4) is never cut
+            // :1490-1503: beta(4) keeps weno_eps, alpha(4) is never cut.  The
+            // real power 3 is the product x*x*x, as the reference build
+            // evaluates it (the cut-off is discontinuous in these values).
             const double tau = fabs(eps - b[3]);
             double a[5];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const double t = pow(1.0 + tau / b[q], 3.0);
+                const double x = 1.0 + tau / b[q];
+                const double t = x * x * x;
                 a[q] = t * t;
             }
             {
-                const double t = pow(1.0 + tau / eps, 3.0);
+                const double x = 1.0 + tau / eps;
+                const double t = x * x * x;
                 a[4] = t * t;
             }
             const double s = a[0] + a[1] + a[2] + a[3] + a[4];

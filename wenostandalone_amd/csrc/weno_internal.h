@@ -7,7 +7,9 @@
 namespace wm {
 
 enum Op { OP_RESHAPE = 0, OP_COPY = 1, OP_WENO = 2, OP_MP = 3 };
-enum Mode { MODE_NONE = 0, MODE_JS = 1, MODE_TENO = 2 };
+// Weight branches of the reference (weno_standalone.f90:552-606), in its
+// order of precedence: wenojs, mapped_weno, wenoz, teno; none -> NaN.
+enum Mode { MODE_NONE = 0, MODE_JS = 1, MODE_TENO = 2, MODE_WENOZ = 3, MODE_MAPPED = 4 };
 enum Dtype { DT_F64 = 0, DT_F32 = 1, DT_BF16 = 2 };
 
 // Per-direction stencil coefficients, in the reference's layouts

@@ -366,12 +366,18 @@ static hipError_t go_weno(const TileArgs& a, const Launch& l, hipStream_t st)
     }
     WM_CASE(3, MODE_NONE)
     WM_CASE(3, MODE_JS)
+    WM_CASE(3, MODE_MAPPED)
+    WM_CASE(3, MODE_WENOZ)
     WM_CASE(5, MODE_NONE)
     WM_CASE(5, MODE_JS)
     WM_CASE(5, MODE_TENO)
+    WM_CASE(5, MODE_MAPPED)
+    WM_CASE(5, MODE_WENOZ)
     WM_CASE(7, MODE_NONE)
     WM_CASE(7, MODE_JS)
     WM_CASE(7, MODE_TENO)
+    WM_CASE(7, MODE_MAPPED)
+    WM_CASE(7, MODE_WENOZ)
 #undef WM_CASE
     return hipErrorInvalidValue;
 }

@@ -78,6 +78,31 @@ def random_coefficients(nc: int, seed: int):
     return polyL, polyR, dL, dR, beta
 
 
+def psd_coefficients(nc: int, order: int, seed: int):
+    """j-varying coefficients whose smoothness indicators are positive
+    semi-definite quadratic forms in the stencil's differences, as every real
+    WENO scheme's are: beta_coef(j, r, :) holds the upper triangle of
+    M = A A^T (A random) in the reference's term order -- WENO3 (m00),
+    WENO5 (m00, 2 m01, m11), WENO7 (m00, 2 m01, 2 m02, m11, 2 m12, m22)
+    (weno_standalone.f90:427-428, :545-550, :1462-1473).  Branches whose
+    weights are linear in 1/beta (WENO-Z) need beta >= eps to be well
+    conditioned; random_coefficients' independent cross terms do not
+    guarantee it."""
+    polyL, polyR, dL, dR, _ = random_coefficients(nc, seed)
+    g = np.random.default_rng([seed, 79, order])
+    n = {3: 1, 5: 2, 7: 3}[order]
+    beta = np.zeros((nc, 5, 6), order="F")
+    for r in range(5):
+        A = g.uniform(-1.0, 1.0, (nc, n, n))
+        M = np.einsum("jab,jcb->jac", A, A)
+        col = 0
+        for a in range(n):
+            for c in range(a, n):
+                beta[:, r, col] = M[:, a, c] * (1.0 if a == c else 2.0)
+                col += 1
+    return polyL, polyR, dL, dR, np.asfortranarray(beta)
+
+
 def seeded_field(m: int, n: int, p: int, v: int, buff: int, seed: int, recipe: int = 0) -> np.ndarray:
     """Seeded synthetic field.
 

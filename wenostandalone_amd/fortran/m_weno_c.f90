@@ -19,6 +19,8 @@ module m_weno_c
         integer(c_int) :: wenojs
         integer(c_int) :: teno
         real(c_double) :: teno_CT
+        integer(c_int) :: mapped_weno
+        integer(c_int) :: wenoz
     end type weno_grid_t
 
     type, bind(C) :: weno_coeffs_t

@@ -29,9 +29,14 @@ module m_weno_standalone
     integer, parameter :: weno_polyn_max = 3
     integer, parameter :: weno_stencils_max = 4
     integer, parameter :: beta_terms_max = 6
-    ! TENO is a compile-time .false. in the reference (:74); here a runtime
-    ! switch (environment WENO_TENO=1)
+    ! The weight branches mapped_weno, wenoz and teno are compile-time
+    ! .false. parameters in the reference (:72-74); here runtime switches
+    ! (environment WENO_MAPPED=1, WENO_Z=1, WENO_TENO=1).  wenoz_q (:75) is
+    ! fixed at 2 as in the reference.
+    logical :: mapped_weno = .false.
+    logical :: wenoz = .false.
     logical :: teno = .false.
+    integer, parameter :: wenoz_q = 2
 
     logical :: weno5_split_kernels = .false.
     logical :: weno5_specialized_combined = .false.
@@ -71,6 +76,8 @@ contains
         g%wenojs = merge(1, 0, wenojs)
         g%teno = merge(1, 0, teno)
         g%teno_CT = teno_CT
+        g%mapped_weno = merge(1, 0, mapped_weno)
+        g%wenoz = merge(1, 0, wenoz)
     end function grid
 
     subroutine s_read_input(input_file)

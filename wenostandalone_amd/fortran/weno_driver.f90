@@ -3,7 +3,8 @@
 !! reference's program weno_test_driver (inputs/weno_standalone.f90:1734-1861),
 !! so `wenoss.sh test`-style checksum parsing works unchanged.
 !!
-!! Extra switches: WENO_TENO=1 (TENO weights when wenojs=F), WENO5_FUSED=1
+!! Extra switches: WENO_MAPPED=1, WENO_Z=1, WENO_TENO=1 (mapped-WENO, WENO-Z
+!! or TENO weights when wenojs=F, in that precedence), WENO5_FUSED=1
 !! (weno5 mode: one pass per sweep from v_vf, order-1 halo included).
 program weno_test_driver
 
@@ -36,6 +37,8 @@ program weno_test_driver
     weno5_specialized_combined = f_get_env_int('WENO5_SPECIALIZED_COMBINED', 0) /= 0
     weno5_fused = f_get_env_int('WENO5_FUSED', 0) /= 0
     teno = f_get_env_int('WENO_TENO', 0) /= 0
+    mapped_weno = f_get_env_int('WENO_MAPPED', 0) /= 0
+    wenoz = f_get_env_int('WENO_Z', 0) /= 0
 
     write(*,'(a)') '==============================================='
     write(*,'(a)') '  Standalone MFC WENO harness (MI355X HIP path)'

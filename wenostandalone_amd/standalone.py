@@ -22,10 +22,12 @@ class WenoStandalone:
 
     def __init__(self, m_cells=199, n_cells=199, p_cells=199, v_size=8, buff_size=6,
                  weno_eps=1e-16, wenojs=True, teno_CT=1e-5, teno=False,
-                 weno5_split_kernels=False, weno5_specialized_combined=False):
+                 weno5_split_kernels=False, weno5_specialized_combined=False,
+                 mapped_weno=False, wenoz=False):
         self.m_cells, self.n_cells, self.p_cells = m_cells, n_cells, p_cells
         self.v_size, self.buff_size = v_size, buff_size
         self.weno_eps, self.wenojs, self.teno_CT, self.teno = weno_eps, wenojs, teno_CT, teno
+        self.mapped_weno, self.wenoz = mapped_weno, wenoz
         self.weno5_split_kernels = weno5_split_kernels
         self.weno5_specialized_combined = weno5_specialized_combined
         check(lib.weno_init(-1))
@@ -34,7 +36,7 @@ class WenoStandalone:
     @property
     def grid(self):
         return make_grid(self.m_cells, self.n_cells, self.p_cells, self.v_size, self.buff_size,
-                         self.weno_eps, self.wenojs, self.teno, self.teno_CT)
+                         self.weno_eps, self.wenojs, self.teno, self.teno_CT, self.mapped_weno, self.wenoz)
 
     def face_shape(self, d):
         ns, nk, nl = fields.dir_extents(self.m_cells, self.n_cells, self.p_cells, self.buff_size, d)
