@@ -1,8 +1,10 @@
 """The WENO5-JS FP64 producer/consumer kernel (weno_sweep5.hip).
 
-weno_reconstruct routes WENO5-JS FP64 launches (no fused limiter, both sides,
-rows of >= 128 outputs) to this kernel; the other cases stay on the general
-sweep kernel.  Checked here, 1e-13 relative (rel_err) throughout:
+weno_reconstruct routes WENO5-JS launches (both sides, rows of >= 96 outputs,
+with or without the fused MP limiter) to this kernel; the other cases stay on
+the general sweep kernel.  Checked here, 1e-13 relative (rel_err) throughout,
+every case also with the limiter fused (WENO_FUSE_MP, against the oracle's
+weno5 followed by its MP):
   * against the oracle on boxes whose three sweeps all take it: field input
     (contiguous x rows loaded in 16-byte pieces, y/z pencils transposed on the
     way into LDS) and reshaped input, core-only and fused halo copy (halo
@@ -101,11 +103,18 @@ def _check_box(g, og, f, dirs=(0, 1, 2), seed=0, split=None):
         rs = DeviceArray.from_host(v_rs)
         cL, cR = oracle.order1_copy(og, d, f)
         eL, eR = oracle.weno(og, 5, d, v_rs, cs, cL, cR)
+        mL, mR = oracle.mp(og, d, v_rs, eL, eR)
         for src, kind in [(vf, wm.SRC_FIELD), (rs, wm.SRC_RESHAPED)]:
             L, R = _run(g, d, src, kind, coef, 0, pre=(cL, cR))
             assert _rel(L, eL, split) <= 1e-13 and _rel(R, eR, split) <= 1e-13, (d, kind, "core")
             L, R = _run(g, d, src, kind, coef, wm.FUSE_COPY)
             assert _rel(L, eL, split) <= 1e-13 and _rel(R, eR, split) <= 1e-13, (d, kind, "copy")
+            assert np.array_equal(L[:3], cL[:3], equal_nan=True)
+            assert np.array_equal(R[-3:], cR[-3:], equal_nan=True)
+            L, R = _run(g, d, src, kind, coef, wm.FUSE_MP, pre=(cL, cR))
+            assert _rel(L, mL, split) <= 1e-13 and _rel(R, mR, split) <= 1e-13, (d, kind, "core+mp")
+            L, R = _run(g, d, src, kind, coef, wm.FUSE_COPY | wm.FUSE_MP)
+            assert _rel(L, mL, split) <= 1e-13 and _rel(R, mR, split) <= 1e-13, (d, kind, "copy+mp")
             assert np.array_equal(L[:3], cL[:3], equal_nan=True)
             assert np.array_equal(R[-3:], cR[-3:], equal_nan=True)
 
@@ -134,6 +143,10 @@ def test_core_only_keeps_arbitrary_halo():
         v_rs = oracle.reshape(og, d, f)
         eL, eR = oracle.weno(og, 5, d, v_rs, cs, pL, pR)
         assert rel_err(L, eL) <= 1e-13 and rel_err(R, eR) <= 1e-13, d
+        L, R = _run(g, d, vf, wm.SRC_FIELD, coef, wm.FUSE_MP, pre=(pL, pR))
+        assert np.array_equal(L[halo], pL[halo]) and np.array_equal(R[halo], pR[halo]), d
+        mL, mR = oracle.mp(og, d, v_rs, eL, eR)
+        assert rel_err(L, mL) <= 1e-13 and rel_err(R, mR) <= 1e-13, d
 
 
 def test_odd_extents_and_partial_tiles():
@@ -157,13 +170,14 @@ def test_matches_general_kernel():
     for d in range(3):
         ns = fields.dir_extents(*BOX, 6, d)[0]
         devs, coef = _coef(fields.harness_coefficients(ns - 6, fields.DIR_OFFSETS[d]))
-        fast = _run(g, d, vf, wm.SRC_FIELD, coef, wm.FUSE_COPY)
-        wm.check(wm.lib.weno_set_tile(5, wm.F64, d, 0, 0, 0, 8 | 2))
-        try:
-            gen = _run(g, d, vf, wm.SRC_FIELD, coef, wm.FUSE_COPY)
-        finally:
-            wm.check(wm.lib.weno_set_tile(5, wm.F64, d, 0, 0, 0, -1))
-        assert rel_err(fast[0], gen[0]) <= 1e-13 and rel_err(fast[1], gen[1]) <= 1e-13, d
+        for flags in (wm.FUSE_COPY, wm.FUSE_COPY | wm.FUSE_MP):
+            fast = _run(g, d, vf, wm.SRC_FIELD, coef, flags)
+            wm.check(wm.lib.weno_set_tile(5, wm.F64, d, 0, 0, 0, 8 | 2))
+            try:
+                gen = _run(g, d, vf, wm.SRC_FIELD, coef, flags)
+            finally:
+                wm.check(wm.lib.weno_set_tile(5, wm.F64, d, 0, 0, 0, -1))
+            assert rel_err(fast[0], gen[0]) <= 1e-13 and rel_err(fast[1], gen[1]) <= 1e-13, (d, flags)
 
 
 @pytest.mark.parametrize("scale,eps", [(1e22, 1e-16), (1e-30, 1e-100)])
@@ -235,6 +249,10 @@ def test_reduced_storage_all_sweeps(state):
             assert rel_err(L, eL) <= 1.2e-7 and rel_err(R, eR) <= 1.2e-7, (d, kind, "copy")
             assert np.array_equal(L[halo], cL[halo].astype(np.float32))
             assert np.array_equal(R[halo], cR[halo].astype(np.float32))
+            L, R = _run(g, d, src, kind, coef, wm.FUSE_COPY | wm.FUSE_MP, dtype=dt)
+            mL, mR = oracle.mp(og, d, v_rs, eL, eR)
+            assert rel_err(L, mL) <= 1.2e-7 and rel_err(R, mR) <= 1.2e-7, (d, kind, "copy+mp")
+            assert np.array_equal(L[halo], cL[halo].astype(np.float32))
 
 
 def test_fp32_storage_matches_general_kernel_and_nan():

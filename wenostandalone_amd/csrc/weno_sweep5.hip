@@ -28,6 +28,12 @@
 //   one barrier per tile hands buffer k+1 to the consumers and buffer k back
 //   to the producer.
 //
+// With MP the monotonicity-preserving limiter (s_preserve_monotonicity,
+// weno_standalone.f90:1544-1615) is applied in the same pass: its 5-point
+// window is the WENO5 window the lane already holds, so the limited faces are
+// written once and the limiter costs arithmetic only (FP64, before the face
+// is rounded to its storage type).
+//
 // Output rows are always written whole.  Outputs outside the reconstructed
 // range (j < 3, j > ns-4) get the order-1 copy with WENO_FUSE_COPY; without
 // it they are rewritten with the values they already hold (the producer loads
@@ -280,7 +286,7 @@ __device__ __forceinline__ long long out_base(const TileArgs& a, int iv, int pa,
 //        of a tile are consecutive along x, loaded as 128-byte row segments
 //        (one pencil per lane, 4 rows per wave instruction) and transposed
 //        into LDS.
-template <typename TS, int LAY, bool COPY, bool PAIR>
+template <typename TS, int LAY, bool COPY, bool PAIR, bool MP>
 __global__ void __attribute__((amdgpu_flat_work_group_size(64, kThreads5), amdgpu_waves_per_eu(4)))
 sweep5_kernel(const TileArgs a)
 {
@@ -505,6 +511,11 @@ sweep5_kernel(const TileArgs a)
                 const double v0 = row[-2], v1 = row[-1], v2 = row[0], v3 = row[1], v4 = row[2];
                 double oL, oR;
                 allok &= cell5<NEWTON>(v0, v1, v2, v3, v4, k5, eps, oL, oR);
+                if constexpr (MP) {
+                    // every lane limits (halo lanes replace their value below)
+                    const double w[5] = {v0, v1, v2, v3, v4};
+                    mp_limit<double>(w, oL, oR);
+                }
                 TO wL = static_cast<TO>(oL), wR = static_cast<TO>(oR);
                 if (halo) {
                     if constexpr (COPY) {
@@ -525,6 +536,10 @@ sweep5_kernel(const TileArgs a)
                     const double* row = cur + 2 * qq * ldj + colc;
                     double oL, oR;
                     cell5_slow(row[-2], row[-1], row[0], row[1], row[2], k5, eps, oL, oR);
+                    if constexpr (MP) {
+                        const double w[5] = {row[-2], row[-1], row[0], row[1], row[2]};
+                        mp_limit<double>(w, oL, oR);
+                    }
                     const unsigned off = s_off[slot][q] + joff;
                     store_face(static_cast<TO>(oL), rL, off);
                     store_face(static_cast<TO>(oR), rR, off);
@@ -537,12 +552,14 @@ sweep5_kernel(const TileArgs a)
 
 // Host side: decides whether a WENO5-JS launch runs on this kernel and sets
 // its tile shape.  Returns the layout (0 or 1), or -1 when the general sweep
-// kernel must be used (other orders/modes, the fused limiter, one-sided
-// output, rows shorter than 96 outputs, odd row lengths or misaligned
-// arrays, arrays too large for 32-bit offsets).
+// kernel must be used (other orders/modes, a fused limiter range other than
+// the reconstructed range, one-sided output, rows shorter than 96 outputs,
+// odd row lengths or misaligned arrays, arrays too large for 32-bit
+// offsets).
 int plan_sweep5(TileArgs& a, Launch& l, const void* src)
 {
-    if (l.op != OP_WENO || l.order != 5 || l.mode != MODE_JS || l.mp || a.sides != 3) return -1;
+    if (l.op != OP_WENO || l.order != 5 || l.mode != MODE_JS || a.sides != 3) return -1;
+    if (l.mp && (a.mp_beg != a.j_beg || a.mp_end != a.j_end)) return -1;
     int in_b, out_b;
     switch (l.dtype) {
     case DT_F64: in_b = 8, out_b = 8; break;
@@ -616,23 +633,28 @@ int plan_sweep5(TileArgs& a, Launch& l, const void* src)
 
 namespace {
 typedef void (*Sweep5Fn)(const TileArgs);
-template <typename TS>
+template <typename TS, bool MP>
 Sweep5Fn pick5(int lay, bool pair, bool copy)
 {
-    return lay == 0 ? (copy ? sweep5_kernel<TS, 0, true, false> : sweep5_kernel<TS, 0, false, false>)
-           : pair   ? (copy ? sweep5_kernel<TS, 1, true, true> : sweep5_kernel<TS, 1, false, true>)
-                    : (copy ? sweep5_kernel<TS, 1, true, false> : sweep5_kernel<TS, 1, false, false>);
+    return lay == 0 ? (copy ? sweep5_kernel<TS, 0, true, false, MP> : sweep5_kernel<TS, 0, false, false, MP>)
+           : pair   ? (copy ? sweep5_kernel<TS, 1, true, true, MP> : sweep5_kernel<TS, 1, false, true, MP>)
+                    : (copy ? sweep5_kernel<TS, 1, true, false, MP> : sweep5_kernel<TS, 1, false, false, MP>);
+}
+template <typename TS>
+Sweep5Fn pick5(int lay, bool pair, bool copy, bool mp)
+{
+    return mp ? pick5<TS, true>(lay, pair, copy) : pick5<TS, false>(lay, pair, copy);
 }
 }  // namespace
 
-hipError_t launch_sweep5(TileArgs a, int lay, int dtype, hipStream_t st)
+hipError_t launch_sweep5(TileArgs a, int lay, int dtype, int mp, hipStream_t st)
 {
     if (a.threads != kThreads5 || 2 * a.jn > kConsumers || a.ldj > 0xffff) return hipErrorInvalidConfiguration;
     const size_t lds = 2 * static_cast<size_t>(kNP) * a.ldj * sizeof(double);
     const bool copy = a.copy_halo != 0, pair = a.pair_b != 0;
-    const Sweep5Fn kern = dtype == DT_F64   ? pick5<double>(lay, pair, copy)
-                          : dtype == DT_F32 ? pick5<float>(lay, pair, copy)
-                                            : pick5<bf16_t>(lay, pair, copy);
+    const Sweep5Fn kern = dtype == DT_F64   ? pick5<double>(lay, pair, copy, mp != 0)
+                          : dtype == DT_F32 ? pick5<float>(lay, pair, copy, mp != 0)
+                                            : pick5<bf16_t>(lay, pair, copy, mp != 0);
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
