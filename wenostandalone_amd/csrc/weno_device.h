@@ -232,6 +232,22 @@ __device__ __forceinline__ double masked_combine(const double (&a)[N], const C (
     return num / (S + extra);
 }
 
+// x**6 for TENO5's (1 + tau/beta)**6._wp (weno_standalone.f90:580-582), which
+// the reference evaluates with the C library's pow.  Here x^2 = p2 + e2 and
+// x^3 = p3 + e3 are carried as unevaluated double-double sums (Dekker's
+// products through FMA), x^6 = (x^3)^2 likewise, and the sum is rounded once:
+// relative error about 2^-100 before that rounding, so the result is x^6
+// correctly rounded (pow's result to within its own < 1 ulp) at about a tenth
+// of pow's instruction count.  When x^6 overflows the plain product (inf) is
+// returned, as pow does.
+__device__ __forceinline__ double pow6(double x)
+{
+    const double p2 = x * x, e2 = fma(x, x, -p2);
+    const double p3 = p2 * x, e3 = fma(p2, x, -p3) + e2 * x;
+    const double p6 = p3 * p3, e6 = fma(p3, p3, -p6) + 2.0 * p3 * e3;
+    return fabs(p6) <= DBL_MAX ? p6 + e6 : p6;
+}
+
 // Mapped WENO (weno_standalone.f90:433-437, :557-570, :1478-1482): the JS
 // weights w_r = a_r / sum(a), a_r = d_r / beta_r^2, mapped through
 //   g_r(w) = (d_r (1 + d_r - 3 w) + w^2) * (w / (d_r^2 + w (1 - 2 d_r)))
@@ -432,11 +448,11 @@ __device__ __forceinline__ void weno5(const C* v, const Cf<5, C>& f, const BT& B
             oL = wenoz_combine<3, 1, C>(bb, tau, dl, pL);
             oR = wenoz_combine<3, 1, C>(bb, tau, dr, pR);
         } else {
-            // :578-606, kept with IEEE divisions and pow
+            // :578-606, with IEEE divisions
             const double tau = fabs(b2 - b0);
-            const double a0 = pow(1.0 + tau / b0, 6.0);
-            const double a1 = pow(1.0 + tau / b1, 6.0);
-            const double a2 = pow(1.0 + tau / b2, 6.0);
+            const double a0 = pow6(1.0 + tau / b0);
+            const double a1 = pow6(1.0 + tau / b1);
+            const double a2 = pow6(1.0 + tau / b2);
             const double s = a0 + a1 + a2;
             const double del0 = (a0 / s < ct) ? 0.0 : 1.0;
             const double del1 = (a1 / s < ct) ? 0.0 : 1.0;
