@@ -39,75 +39,187 @@ typedef __attribute__((address_space(3))) const long long LdsI64;
 typedef __attribute__((address_space(1))) double GlbF64;
 typedef __attribute__((address_space(1))) const double GlbCF64;
 
-// tile, table and s_out live in LDS; the address-space-qualified pointers
-// keep their accesses ds_read/ds_write when this function is not inlined
-template <int ORDER, int MODE, bool MP>
-__device__ void batch_compute(const TileArgs& a, const LdsF64* tile, LdsMutF64* table, const LdsI64* s_out,
-                              int lo, int c0, int c1, int jlo, int jhi, bool has_core, int* counter, int& next)
+// Geometry of a block's items along the sweep: input rows [lo, lo+nJ) land in
+// LDS, lanes cover outputs [jlo, jhi], the core (reconstructed) range is
+// [c0, c1].
+struct ItemGeom {
+    int lo, nJ, jlo, jhi, c0, c1;
+    bool has_core;
+};
+
+__device__ __forceinline__ ItemGeom item_geom(const TileArgs& a, int R)
 {
+    ItemGeom g;
+    g.c0 = a.j_beg;
+    g.c1 = a.j_end;
+    g.has_core = g.c0 <= g.c1;
+    int hi;
+    if (a.copy_halo) {
+        g.lo = min(a.J0f, g.c0 + a.J0f - R);
+        hi = max(a.out_len - 1 + a.J0f, g.c1 + a.J0f + R);
+        g.jlo = 0;
+        g.jhi = a.out_len - 1;
+    } else {
+        g.lo = g.c0 + a.J0f - R;
+        hi = g.c1 + a.J0f + R;
+        g.jlo = g.c0;
+        g.jhi = g.c1;
+    }
+    g.nJ = hi - g.lo + 1;
+    return g;
+}
+
+// Item t of a block: pencil addresses into s_in/s_out (threads < np), a
+// barrier, then the tile into LDS.  Element e of the tile is (pencil q, row
+// J): e = q*nJ + J when rows are contiguous (s_j == 1), e = J*np + q
+// otherwise, so that consecutive lanes read consecutive addresses.  Each lane
+// takes elements tid, tid+T, ...; (q, J) advance by the divmod of T instead of
+// dividing per element, and kFill loads are issued before the first LDS
+// store, so a lane waits for one memory latency per kFill elements instead of
+// one per element.  The caller's next barrier publishes the tile.
+__device__ __forceinline__ void fill_tile(const TileArgs& a, const ItemGeom& g, int t, double* tile,
+                                          long long* s_in, long long* s_out)
+{
+    const int tid = threadIdx.x, T = blockDim.x;
+    const int np = a.np, ldj = a.ldj, nJ = g.nJ;
+    if (tid < np) {
+        const int iv = t / a.tiles_per_var;
+        const int P = (t - iv * a.tiles_per_var) * np + tid;
+        if (P < a.na * a.nb) {
+            const int pa = P % a.na, pb = P / a.na;
+            s_in[tid] = a.src_off + pa * a.s_a + pb * a.s_b + iv * a.s_i + g.lo * a.s_j;
+            const int k = a.a_is_k ? pa : pb;
+            const int l = a.a_is_k ? pb : pa;
+            s_out[tid] = k * a.o_k + l * a.o_l + iv * a.o_i;
+        } else {
+            s_in[tid] = -1;
+            s_out[tid] = -1;
+        }
+    }
+    __syncthreads();
+    GlbCF64* const src = (GlbCF64*)a.src;
+    constexpr int kFill = 8;
+    const int n = np * nJ;
+    const bool rows = a.s_j == 1;
+    const int inner = rows ? nJ : np;      // extent of the fast index
+    const int d_hi = T / inner, d_lo = T - d_hi * inner;
+    int hi_i = tid / inner, lo_i = tid - hi_i * inner;
+    for (int e0 = tid; e0 < n; e0 += kFill * T) {
+        double r[kFill];
+        int dst[kFill];
+#pragma unroll
+        for (int u = 0; u < kFill; ++u) {
+            const int q = rows ? hi_i : lo_i, J = rows ? lo_i : hi_i;
+            dst[u] = -1;
+            r[u] = 0.0;
+            if (e0 + u * T < n) {
+                const long long b = s_in[q];
+                dst[u] = q * ldj + J;
+                if (b >= 0) r[u] = src[b + J * a.s_j];
+            }
+            lo_i += d_lo;
+            hi_i += d_hi;
+            if (lo_i >= inner) {
+                lo_i -= inner;
+                ++hi_i;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kFill; ++u)
+            if (dst[u] >= 0) tile[dst[u]] = r[u];
+    }
+}
+
+// A run of consecutive items of one block (descriptor D), starting at `item`.
+// The block's coefficients are loaded once for the run: the candidate
+// polynomial coefficients into registers, beta and d into an LDS table
+// [lane][entry] after the tile.  Per item: pencil addresses, tile, one
+// barrier, the claim of the next item (thread 0; its round trip overlaps the
+// reconstruction), reconstruction and stores, one barrier after which every
+// thread reads the claimed item from s_item.  Returns when that item belongs
+// to another block (or there are no more items); s_item holds it.
+template <int ORDER, int MODE, bool MP>
+__device__ void batch_block(const BatchDesc& D, int item, double* tile, long long* s_in, long long* s_out,
+                            int* s_item, int* counter, int& next)
+{
+    const TileArgs& a = D.a;
     constexpr int RW = ORDER / 2;
     constexpr int R = MP ? (RW > 2 ? RW : 2) : RW;
     constexpr int W = 2 * R + 1;
     constexpr int NS = BetaTraits<ORDER>::NS, NB = BetaTraits<ORDER>::N, NR = BetaTraits<ORDER>::NR;
     constexpr int LDB = BetaTraits<ORDER>::LD;
+    const ItemGeom g = item_geom(a, R);
+    const int item_end = D.item0 + a.tiles_per_var * a.nv;
     const int tid = threadIdx.x;
     const int gq = tid / a.jn;
     const int jj = tid - gq * a.jn;
-    const int j = jlo + jj;
-    const bool active = gq < a.groups && j <= jhi;
-    const bool core = active && has_core && j >= c0 && j <= c1;
+    const int j = g.jlo + jj;
+    const bool active = gq < a.groups && j <= g.jhi;
+    const bool core = active && g.has_core && j >= g.c0 && j <= g.c1;
     const bool do_mp = MP && core && j >= a.mp_beg && j <= a.mp_end;
+    LdsMutF64* const table = (LdsMutF64*)(tile + a.np * a.ldj);
     Cf<ORDER, double> f;
     if (core) {
         int jc = j - 3;
         jc = jc < 0 ? 0 : (jc > a.c.nc - 1 ? a.c.nc - 1 : jc);
         load_cf(a.c, jc, f);
-        if (gq == 0) {
+        if (gq == 0) {   // the groups share the table (same j per lane index)
+            const GlbCoef *beta = gcoef(a.c.beta), *dL = gcoef(a.c.dL), *dR = gcoef(a.c.dR);
+#pragma unroll
             for (int t = 0; t < NB; ++t) {
                 const int r = t / NS, s = t - (t / NS) * NS;
-                table[jj * LDB + t] = a.c.beta[jc + a.c.nc * (r + 5 * s)];
+                table[jj * LDB + t] = beta[jc + a.c.nc * (r + 5 * s)];
             }
+#pragma unroll
             for (int r = 0; r < NR; ++r) {
-                table[jj * LDB + NB + r] = a.c.dL[r + 5 * jc];
-                table[jj * LDB + NB + NR + r] = a.c.dR[r + 5 * jc];
+                table[jj * LDB + NB + r] = dL[r + 5 * jc];
+                table[jj * LDB + NB + NR + r] = dR[r + 5 * jc];
             }
         }
     }
-    __syncthreads();
-    // claim the next item now: its round trip overlaps the reconstruction
-    // (thread 0's coefficient loads have completed, its stores come later)
-    if (threadIdx.x == 0) next = atomicAdd(counter, 1);
-    if (!active) return;
     GlbF64* const outL = (GlbF64*)a.outL;
     GlbF64* const outR = (GlbF64*)a.outR;
-    const int col = j + a.J0f - lo;
-    const BetaRef<NS> B{table + jj * LDB, NB, NR};
-    for (int qq = 0; qq < a.npg; ++qq) {
-        const int q = gq * a.npg + qq;
-        const long long ob = s_out[q];
-        if (ob < 0) continue;
-        const LdsF64* row = tile + q * a.ldj + col;
-        if (!core) {
-            outL[ob + j] = row[0];
-            outR[ob + j] = row[0];
-            continue;
-        }
-        double v[W];
+    const int col = j + a.J0f - g.lo;
+    const BetaRef<NS> B{(const LdsF64*)table + jj * LDB, NB, NR};
+    const LdsF64* const ltile = lds_ptr(tile);
+    const LdsI64* const lout = (const LdsI64*)s_out;
+    for (;;) {
+        fill_tile(a, g, item - D.item0, tile, s_in, s_out);
+        __syncthreads();
+        if (tid == 0) next = atomicAdd(counter, 1);
+        if (active) {
+            for (int qq = 0; qq < a.npg; ++qq) {
+                const int q = gq * a.npg + qq;
+                const long long ob = lout[q];
+                if (ob < 0) continue;
+                const LdsF64* row = ltile + q * a.ldj + col;
+                if (!core) {
+                    outL[ob + j] = row[0];
+                    outR[ob + j] = row[0];
+                    continue;
+                }
+                double v[W];
 #pragma unroll
-        for (int o = 0; o < W; ++o) v[o] = row[o - R];
-        double oL, oR;
-        const double* vc = v + (R - RW);
-        if constexpr (ORDER == 3)
-            weno3<MODE, double>(vc, f, B, a.eps, a.teno_ct, oL, oR);
-        else if constexpr (ORDER == 5)
-            weno5<MODE, double>(vc, f, B, a.eps, a.teno_ct, oL, oR);
-        else
-            weno7<MODE, double>(vc, f, B, a.eps, a.teno_ct, oL, oR);
-        if constexpr (MP) {
-            if (do_mp) mp_limit<double>(v + (R - 2), oL, oR);
+                for (int o = 0; o < W; ++o) v[o] = row[o - R];
+                double oL, oR;
+                const double* vc = v + (R - RW);
+                if constexpr (ORDER == 3)
+                    weno3<MODE, double>(vc, f, B, a.eps, a.teno_ct, oL, oR);
+                else if constexpr (ORDER == 5)
+                    weno5<MODE, double>(vc, f, B, a.eps, a.teno_ct, oL, oR);
+                else
+                    weno7<MODE, double>(vc, f, B, a.eps, a.teno_ct, oL, oR);
+                if constexpr (MP) {
+                    if (do_mp) mp_limit<double>(v + (R - 2), oL, oR);
+                }
+                outL[ob + j] = oL;
+                outR[ob + j] = oR;
+            }
         }
-        outL[ob + j] = oL;
-        outR[ob + j] = oR;
+        if (tid == 0) *s_item = next;
+        __syncthreads();
+        item = *s_item;
+        if (item >= item_end) return;
     }
 }
 
@@ -125,82 +237,31 @@ __global__ void __launch_bounds__(kBatchThreads)
     const bool lds_search = ndesc <= kBatchLdsDescs;
     if (lds_search)
         for (int i = tid; i < ndesc; i += T) s_item0[i] = descs[i].item0;
-    // the next item is claimed while the current one is computed, so the
-    // atomic's round trip is hidden (thread 0 holds it; batch_compute claims)
+    // thread 0 holds the claimed item in `next` (batch_block claims the
+    // following one while it reconstructs)
     int next = 0;
-    if (tid == 0) next = atomicAdd(counter, 1);
-    __syncthreads();
-
+    if (tid == 0) {
+        next = atomicAdd(counter, 1);
+        s_item = next;
+    }
     for (;;) {
+        // the block of the claimed item (s_item, written by thread 0 before
+        // the barrier that every thread passed since its last read)
         if (tid == 0) {
-            const int it = next;
             int lo = 0, hi = ndesc - 1;
             while (lo < hi) {
                 const int mid = (lo + hi + 1) >> 1;
-                if ((lds_search ? s_item0[mid] : descs[mid].item0) <= it) lo = mid;
+                if ((lds_search ? s_item0[mid] : descs[mid].item0) <= next) lo = mid;
                 else hi = mid - 1;
             }
-            s_item = it;
             s_desc = lo;
         }
         __syncthreads();
         const int item = s_item;
         if (item >= total) break;
         const BatchDesc& D = descs[s_desc];
-        const TileArgs& a = D.a;
         const int order = D.order, mode = D.mode;
-        const int t = item - D.item0;
-        const int np = a.np, ldj = a.ldj;
-        const int RW = order / 2;
-        const int R = MP ? (RW > 2 ? RW : 2) : RW;
-        const int c0 = a.j_beg, c1 = a.j_end;
-        const bool has_core = c0 <= c1;
-        int lo, hi, jlo, jhi;
-        if (a.copy_halo) {
-            lo = min(a.J0f, c0 + a.J0f - R);
-            hi = max(a.out_len - 1 + a.J0f, c1 + a.J0f + R);
-            jlo = 0;
-            jhi = a.out_len - 1;
-        } else {
-            lo = c0 + a.J0f - R;
-            hi = c1 + a.J0f + R;
-            jlo = c0;
-            jhi = c1;
-        }
-        const int nJ = hi - lo + 1;
-        if (tid < np) {
-            const int iv = t / a.tiles_per_var;
-            const int P = (t - iv * a.tiles_per_var) * np + tid;
-            if (P < a.na * a.nb) {
-                const int pa = P % a.na, pb = P / a.na;
-                s_in[tid] = a.src_off + pa * a.s_a + pb * a.s_b + iv * a.s_i + lo * a.s_j;
-                const int k = a.a_is_k ? pa : pb;
-                const int l = a.a_is_k ? pb : pa;
-                s_out[tid] = k * a.o_k + l * a.o_l + iv * a.o_i;
-            } else {
-                s_in[tid] = -1;
-                s_out[tid] = -1;
-            }
-        }
-        __syncthreads();
-        GlbCF64* const src = (GlbCF64*)a.src;
-        for (int e = tid; e < np * nJ; e += T) {
-            int q, J;
-            if (a.s_j == 1) {
-                q = e / nJ;
-                J = e - q * nJ;
-            } else {
-                J = e / np;
-                q = e - J * np;
-            }
-            const long long b = s_in[q];
-            tile[q * ldj + J] = b >= 0 ? src[b + J * a.s_j] : 0.0;
-        }
-        __syncthreads();
-        double* const table = tile + np * ldj;
-#define WM_BATCH(O, M)                                                                                      \
-    batch_compute<O, M, MP>(a, lds_ptr(tile), (LdsMutF64*)table, (const LdsI64*)s_out, lo, c0, c1, jlo, jhi, \
-                            has_core, counter, next)
+#define WM_BATCH(O, M) batch_block<O, M, MP>(D, item, tile, s_in, s_out, &s_item, counter, next)
         if (order == 5) {
             if (mode == MODE_JS) WM_BATCH(5, MODE_JS);
             else if (mode == MODE_TENO) WM_BATCH(5, MODE_TENO);
@@ -220,7 +281,6 @@ __global__ void __launch_bounds__(kBatchThreads)
             else WM_BATCH(3, MODE_NONE);
         }
 #undef WM_BATCH
-        __syncthreads();
     }
 }
 

@@ -66,6 +66,13 @@ __device__ __forceinline__ double fast_div(double n, double d)
 // ---------------------------------------------------------------------------
 // per-j coefficients (reference layouts: poly (nc,5,3), d (5,nc), beta (nc,5,6))
 // ---------------------------------------------------------------------------
+// The coefficient arrays are device global memory.  Read through
+// address-space-1 pointers they are global loads; through a generic pointer
+// they would be flat loads, which also count against the LDS wait counter, so
+// every later LDS wait would wait for them too.
+typedef __attribute__((address_space(1))) const double GlbCoef;
+__device__ __forceinline__ const GlbCoef* gcoef(const double* p) { return (const GlbCoef*)p; }
+
 template <int ORDER, typename C>
 struct Cf;
 template <typename C>
@@ -85,22 +92,24 @@ template <typename C>
 __device__ __forceinline__ void load_cf(const Coefs& c, int jc, Cf<3, C>& f)
 {
     const int nc = c.nc;
+    const GlbCoef *pl = gcoef(c.polyL), *pr = gcoef(c.polyR);
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
-        f.pl[r] = static_cast<C>(c.polyL[jc + nc * r]);
-        f.pr[r] = static_cast<C>(c.polyR[jc + nc * r]);
+        f.pl[r] = static_cast<C>(pl[jc + nc * r]);
+        f.pr[r] = static_cast<C>(pr[jc + nc * r]);
     }
 }
 template <typename C>
 __device__ __forceinline__ void load_cf(const Coefs& c, int jc, Cf<5, C>& f)
 {
     const int nc = c.nc;
+    const GlbCoef *pl = gcoef(c.polyL), *pr = gcoef(c.polyR);
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
-            f.pl[r][s] = static_cast<C>(c.polyL[jc + nc * (r + 5 * s)]);
-            f.pr[r][s] = static_cast<C>(c.polyR[jc + nc * (r + 5 * s)]);
+            f.pl[r][s] = static_cast<C>(pl[jc + nc * (r + 5 * s)]);
+            f.pr[r][s] = static_cast<C>(pr[jc + nc * (r + 5 * s)]);
         }
     }
 }
@@ -108,12 +117,13 @@ template <typename C>
 __device__ __forceinline__ void load_cf(const Coefs& c, int jc, Cf<7, C>& f)
 {
     const int nc = c.nc;
+    const GlbCoef *pl = gcoef(c.polyL), *pr = gcoef(c.polyR);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
-            f.pl[r][s] = static_cast<C>(c.polyL[jc + nc * (r + 5 * s)]);
-            f.pr[r][s] = static_cast<C>(c.polyR[jc + nc * (r + 5 * s)]);
+            f.pl[r][s] = static_cast<C>(pl[jc + nc * (r + 5 * s)]);
+            f.pr[r][s] = static_cast<C>(pr[jc + nc * (r + 5 * s)]);
         }
     }
 }
@@ -161,12 +171,12 @@ struct BetaRegs {
 #pragma unroll
         for (int t = 0; t < N; ++t) {
             const int rr = t / NS, ss = t - (t / NS) * NS;
-            b[t] = c.beta[jc + c.nc * (rr + 5 * ss)];
+            b[t] = gcoef(c.beta)[jc + c.nc * (rr + 5 * ss)];
         }
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
-            l[q] = c.dL[q + 5 * jc];
-            r[q] = c.dR[q + 5 * jc];
+            l[q] = gcoef(c.dL)[q + 5 * jc];
+            r[q] = gcoef(c.dR)[q + 5 * jc];
         }
     }
     __device__ __forceinline__ double operator()(int rr, int ss) const { return b[rr * NS + ss]; }
