@@ -230,6 +230,24 @@ __global__ void __launch_bounds__(kVjpThreads) vjp_kernel(const TileArgs a, doub
     const double* const gL = static_cast<const double*>(a.outL);
     const double* const gR = static_cast<const double*>(a.outR);
 
+    // this lane's tile element s (the same for every tile): e = tid + s*T is
+    // (pencil q, row J) with e = q*nJ + J for contiguous rows, e = J*np + q
+    // otherwise (consecutive lanes read consecutive addresses); packed as
+    // (q << 16) | J, -1 for none.  Recomputed where used: held in registers
+    // for the whole launch it would push WENO7 past 256 VGPRs.
+    auto slot = [&](int s) -> int {
+        const int e = tid + s * T;
+        if (s >= a.slots || e >= np * nJ) return -1;
+        int q, J;
+        if (a.s_j == 1) {
+            q = e / nJ;
+            J = e - q * nJ;
+        } else {
+            J = e / np;
+            q = e - J * np;
+        }
+        return (q << 16) | J;
+    };
     for (int t = t_first; t < t_end; t += nbx) {
         if (tid < np) {
             const int iv = t / a.tiles_per_var;
@@ -246,20 +264,34 @@ __global__ void __launch_bounds__(kVjpThreads) vjp_kernel(const TileArgs a, doub
             }
         }
         __syncthreads();
-        // load the input tile
-        for (int s = 0; s < kVjpSlots; ++s) {
-            const int e = tid + s * T;
-            if (s >= a.slots || e >= np * nJ) break;
-            int q, J;
-            if (a.s_j == 1) {
-                q = e / nJ;
-                J = e - q * nJ;
-            } else {
-                J = e / np;
-                q = e - J * np;
+        // load the input tile: every slot's load is issued before the first
+        // LDS store, so a lane waits for one memory latency per tile
+        {
+            double r[kVjpSlots];
+            int sl[kVjpSlots];
+#pragma unroll
+            for (int s = 0; s < kVjpSlots; ++s) {
+                r[s] = 0.0;
+                sl[s] = slot(s);
+                if (sl[s] >= 0) {
+                    const long long b = s_in[sl[s] >> 16];
+                    if (b >= 0) r[s] = src[b + static_cast<long long>(sl[s] & 0xffff) * a.s_j];
+                }
             }
-            const long long b = s_in[q];
-            tile[q * ldj + J] = b >= 0 ? src[b + J * a.s_j] : 0.0;
+#pragma unroll
+            for (int s = 0; s < kVjpSlots; ++s)
+                if (sl[s] >= 0) tile[(sl[s] >> 16) * ldj + (sl[s] & 0xffff)] = r[s];
+        }
+        // WENO5: the first pencil's cotangents (each cell fetches the next
+        // pencil's while it computes, so their latency overlaps the
+        // arithmetic; WENO7 has no registers to spare and loads them per cell)
+        double gl_n = 0.0, gr_n = 0.0;
+        if (ALL && core) {
+            const long long ob = s_out[gq * a.npg];
+            if (ob >= 0) {
+                gl_n = gL[ob + j];
+                gr_n = gR[ob + j];
+            }
         }
         __syncthreads();
         // contributions of the outputs, one barrier, the sums per input
@@ -273,12 +305,26 @@ __global__ void __launch_bounds__(kVjpThreads) vjp_kernel(const TileArgs a, doub
                 double* const cb = cbuf + (ALL ? q : gq) * W * jn;
                 if (core) {
                     const long long ob = s_out[q];
+                    double gl = gl_n, gr = gr_n;
+                    if constexpr (ALL) {
+                        gl_n = gr_n = 0.0;
+                        if (qq + 1 < a.npg) {
+                            const long long obn = s_out[q + 1];
+                            if (obn >= 0) {
+                                gl_n = gL[obn + j];
+                                gr_n = gR[obn + j];
+                            }
+                        }
+                    } else if (ob >= 0) {
+                        gl = gL[ob + j];
+                        gr = gR[ob + j];
+                    }
                     double c[W];
                     if (ob >= 0) {
                         const LdsF64* bp = lds_ptr(beta_lds + jj * LDB);
                         asm volatile("" : "+v"(bp));
                         const BetaRef<NS> Bref{bp, NB, NR};
-                        weno_vjp_cell<ORDER>(tile + q * ldj + jj, f, Bref, a.eps, gL[ob + j], gR[ob + j], c);
+                        weno_vjp_cell<ORDER>(tile + q * ldj + jj, f, Bref, a.eps, gl, gr, c);
                     } else {
 #pragma unroll
                         for (int o = 0; o < W; ++o) c[o] = 0.0;
@@ -307,20 +353,29 @@ __global__ void __launch_bounds__(kVjpThreads) vjp_kernel(const TileArgs a, doub
             }
             __syncthreads();
         }
-        // accumulate the tile's gradient with the coalesced load mapping
-        for (int s = 0; s < kVjpSlots; ++s) {
-            const int e = tid + s * T;
-            if (s >= a.slots || e >= np * nJ) break;
-            int q, J;
-            if (a.s_j == 1) {
-                q = e / nJ;
-                J = e - q * nJ;
-            } else {
-                J = e / np;
-                q = e - J * np;
+        // accumulate the tile's gradient with the coalesced load mapping: all
+        // reads of gv first, then the sums and stores (each input element
+        // belongs to one pencil, so no other lane or workgroup touches it)
+        {
+            double g[kVjpSlots];
+            int sl[kVjpSlots];
+#pragma unroll
+            for (int s = 0; s < kVjpSlots; ++s) {
+                g[s] = 0.0;
+                sl[s] = slot(s);
+                if (sl[s] >= 0) {
+                    const long long b = s_in[sl[s] >> 16];
+                    if (b >= 0) g[s] = grad[b + static_cast<long long>(sl[s] & 0xffff) * a.s_j];
+                }
             }
-            const long long b = s_in[q];
-            if (b >= 0) grad[b + J * a.s_j] += gtile[q * ldj + J];
+#pragma unroll
+            for (int s = 0; s < kVjpSlots; ++s) {
+                if (sl[s] >= 0) {
+                    const int q = sl[s] >> 16, J = sl[s] & 0xffff;
+                    const long long b = s_in[q];
+                    if (b >= 0) grad[b + static_cast<long long>(J) * a.s_j] = g[s] + gtile[q * ldj + J];
+                }
+            }
         }
         __syncthreads();
     }
@@ -328,6 +383,8 @@ __global__ void __launch_bounds__(kVjpThreads) vjp_kernel(const TileArgs a, doub
 
 hipError_t launch_vjp(TileArgs a, int order, double* grad, hipStream_t st)
 {
+    // tile elements are packed as (pencil << 16) | row
+    if (a.ldj > 0xffff || a.np > kVjpMaxNP || a.slots > kVjpSlots) return hipErrorInvalidConfiguration;
     const size_t lds = (2 * static_cast<size_t>(a.np) * a.ldj +
                         static_cast<size_t>(order == 5 ? a.np : a.groups) * order * a.jn +
                         static_cast<size_t>(order == 5 ? BetaTraits<5>::LD : BetaTraits<7>::LD) * a.jn) *
