@@ -182,6 +182,7 @@ void output_map(const Dims& d, int row_len, TileArgs& a)
 // persistent loop with the coefficients in registers and no prefetch, in
 // 448-thread workgroups (two lane groups of 206 + 2 workgroups per CU).
 constexpr int kDefaultVariant = 2;
+constexpr int kDefaultWeno7Threads = 256;
 constexpr int kDefaultWeno5Threads = 448;
 
 // Pencils per tile, j-chunking and LDS row stride for one launch.
@@ -210,6 +211,13 @@ void plan_tiles(TileArgs& a, Launch& l, int np_override, int chunk_override, int
     int G = 1, T = 64 * ((jn + 63) / 64);
     double best = double(jn) / T;
     if (threads_cap <= 0 && l.op == OP_WENO && l.order == 5) threads_cap = kDefaultWeno5Threads;
+    // FP64 WENO7 with its coefficients in registers (~200-250 VGPRs): two
+    // 256-thread workgroups per CU hold 8 waves where one 448-thread
+    // workgroup holds 7, and need no LDS coefficient table (measured 787 ->
+    // 755 us per 200^3 x 8 sweep)
+    if (threads_cap <= 0 && l.op == OP_WENO && l.order == 7 && l.dtype == DT_F64 &&
+        (((prefetch < 0 ? kDefaultVariant : prefetch) >> 1) & 1))
+        threads_cap = kDefaultWeno7Threads;
     const int tmax = threads_cap > 0 ? std::min(threads_cap, max_threads(l.op, l.order))
                                      : max_threads(l.op, l.order);
     auto search = [&](int cap) {
@@ -243,8 +251,9 @@ void plan_tiles(TileArgs& a, Launch& l, int np_override, int chunk_override, int
     a.jn = jn;
     a.threads = T;
     a.slots = static_cast<int>((static_cast<long long>(a.np) * ldj + T - 1) / T);
-    // variant bits: 1 prefetch, 2 coefficients in registers (WENO5), 4 one
-    // tile per workgroup; -1 = automatic
+    // variant bits: 1 prefetch, 2 coefficients in registers (WENO5; WENO7
+    // with FP64 state: the FP32/BF16 WENO7 kernels are held to 128 VGPRs),
+    // 4 one tile per workgroup; -1 = automatic
     const int variant = prefetch < 0 ? kDefaultVariant : prefetch;
     a.prefetch = variant & 1;
     l.creg = (variant >> 1) & 1;

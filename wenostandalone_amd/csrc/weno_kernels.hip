@@ -358,7 +358,7 @@ static hipError_t go_weno(const TileArgs& a, const Launch& l, hipStream_t st)
 {
 #define WM_CASE(O, M)                                                             \
     if (l.order == O && l.mode == M) {                                            \
-        if (O == 5 && l.creg)                                                     \
+        if ((O == 5 || (O == 7 && sizeof(TIN) == 8)) && l.creg)                  \
             return l.mp ? go<OP_WENO, O, M, true, TIN, TOUT, true>(a, l, st)      \
                         : go<OP_WENO, O, M, false, TIN, TOUT, true>(a, l, st);    \
         return l.mp ? go<OP_WENO, O, M, true, TIN, TOUT>(a, l, st)                \
