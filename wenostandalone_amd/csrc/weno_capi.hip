@@ -183,6 +183,7 @@ void output_map(const Dims& d, int row_len, TileArgs& a)
 // 448-thread workgroups (two lane groups of 206 + 2 workgroups per CU).
 constexpr int kDefaultVariant = 2;
 constexpr int kDefaultWeno7Threads = 256;
+constexpr int kDefaultWeno3Threads = 512;
 constexpr int kDefaultWeno5Threads = 448;
 
 // Pencils per tile, j-chunking and LDS row stride for one launch.
@@ -218,6 +219,10 @@ void plan_tiles(TileArgs& a, Launch& l, int np_override, int chunk_override, int
     if (threads_cap <= 0 && l.op == OP_WENO && l.order == 7 && l.dtype == DT_F64 &&
         (((prefetch < 0 ? kDefaultVariant : prefetch) >> 1) & 1))
         threads_cap = kDefaultWeno7Threads;
+    // FP64 WENO3: two lane groups (448 threads) rather than the three (640)
+    // the lane-utilisation search would pick (measured 497 -> 430 us per
+    // 200^3 x 8 sweep)
+    if (threads_cap <= 0 && l.op == OP_WENO && l.order == 3 && l.dtype == DT_F64) threads_cap = kDefaultWeno3Threads;
     const int tmax = threads_cap > 0 ? std::min(threads_cap, max_threads(l.op, l.order))
                                      : max_threads(l.op, l.order);
     auto search = [&](int cap) {
