@@ -381,7 +381,7 @@ int weno_batched_autotune(const weno_block* blocks, int nblocks, int flags, int*
     float best_ms = 1e30f;
     BatchTune bt;
     for (int th : {256, 512})
-        for (int np : {8, 16, 32}) {
+        for (int np : {8, 16, 32, 64}) {
             BatchPlan plans[3];
             bool ok = true;
             for (int d = 0; d < 3 && ok; ++d) ok = plan_batch(blocks, nblocks, d, flags, th, np, plans[d]) == WENO_OK;
