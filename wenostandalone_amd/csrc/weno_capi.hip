@@ -185,6 +185,7 @@ constexpr int kDefaultVariant = 2;
 constexpr int kDefaultWeno7Threads = 256;
 constexpr int kDefaultWeno3Threads = 512;
 constexpr int kDefaultWeno5Threads = 448;
+constexpr int kDefaultMpThreads = 512;
 
 // Pencils per tile, j-chunking and LDS row stride for one launch.
 void plan_tiles(TileArgs& a, Launch& l, int np_override, int chunk_override, int threads_cap, int prefetch)
@@ -223,6 +224,10 @@ void plan_tiles(TileArgs& a, Launch& l, int np_override, int chunk_override, int
     // the lane-utilisation search would pick (measured 497 -> 430 us per
     // 200^3 x 8 sweep)
     if (threads_cap <= 0 && l.op == OP_WENO && l.order == 3 && l.dtype == DT_F64) threads_cap = kDefaultWeno3Threads;
+    // FP64 standalone limiter: two lane groups (448 threads) rather than the
+    // four (832) the lane-utilisation search would pick (measured 624 -> 609
+    // us per 200^3 x 8 sweep)
+    if (threads_cap <= 0 && l.op == OP_MP && l.dtype == DT_F64) threads_cap = kDefaultMpThreads;
     const int tmax = threads_cap > 0 ? std::min(threads_cap, max_threads(l.op, l.order))
                                      : max_threads(l.op, l.order);
     auto search = [&](int cap) {
