@@ -142,3 +142,32 @@ def test_vjp_central_differences_16cube(order):
         core = (slice(3, ns - 3), slice(None), slice(None), slice(None))
         fd = (np.sum(gL[core] * (Lp - Lm)[core]) + np.sum(gR[core] * (Rp - Rm)[core])) / (2 * h)
         assert abs(grad[X, Y, Z, 0] - fd) <= 1e-6 * max(1.0, abs(fd)), (X, Y, Z, grad[X, Y, Z, 0], fd)
+
+
+@pytest.mark.parametrize("order", [5, 7])
+@pytest.mark.parametrize("m,n,p,v,dirs", [
+    (47, 47, 47, 8, (0, 1, 2)),   # 256-thread workgroups, two per CU, ~4 tiles each
+    (127, 59, 59, 4, (0,)),       # 135-point rows: 448-thread workgroups, ~6 tiles each
+])
+def test_vjp_many_tiles_per_workgroup(order, m, n, p, v, dirs):
+    """Launches where every workgroup runs several tiles in turn (the WENO5
+    path loads tile t+1 while it reconstructs tile t), against the oracle."""
+    b = 6
+    g = wm.make_grid(m, n, p, v=v, buff=b)
+    og = oracle.grid(m, n, p, v=v, buff=b)
+    f = fields.seeded_field(m, n, p, v, b, seed=90 + order)
+    for d in dirs:
+        print(f"vjp many tiles: order {order} box {m}x{n}x{p}x{v} dir {d}", flush=True)
+        ns, nk, nl = fields.dir_extents(m, n, p, b, d)
+        cs = fields.harness_coefficients(ns - 6, fields.DIR_OFFSETS[d])
+        devs = [DeviceArray.from_host(c) for c in cs]
+        coef = wm.Coeffs(*[x.ptr for x in devs])
+        gL, gR = cotangents((ns, nk, nl, v), seed=20 + d)
+        v_rs = oracle.reshape(og, d, f)
+        ref_rs = oracle.vjp(og, order, d, v_rs, cs, gL, gR)
+        dgL, dgR = DeviceArray.from_host(gL), DeviceArray.from_host(gR)
+        rs = DeviceArray.from_host(v_rs)
+        gv = DeviceArray.from_host(np.full(v_rs.shape, 0.25, order="F"))
+        wm.check(wm.lib.weno_reconstruct_vjp(C.byref(g), order, d, wm.SRC_RESHAPED, rs.vp, C.byref(coef),
+                                             dgL.vp, dgR.vp, gv.vp))
+        assert norm_err(gv.download() - 0.25, ref_rs) <= 1e-10, d

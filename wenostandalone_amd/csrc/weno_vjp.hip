@@ -172,6 +172,13 @@ __device__ __forceinline__ void weno_vjp_cell(const double* v, const Cf<ORDER, d
 }
 
 // a.src: v; a.outL/a.outR: gL/gR (read); grad: gv (input layout, accumulated)
+//
+// WENO5 pipelines the tiles of a workgroup: the loads of tile t's gradient
+// and of tile t+1's input are issued before tile t's arithmetic and land in
+// registers (the input in a second LDS tile buffer after the sums), so one
+// memory round trip per tile overlaps the arithmetic; tile addresses rotate
+// through a ring of three.  WENO7 (about 255 VGPRs) has no registers for the
+// loads in flight and runs the tile stages in sequence.
 template <int ORDER>
 __global__ void __launch_bounds__(kVjpThreads) vjp_kernel(const TileArgs a, double* __restrict__ grad)
 {
@@ -181,16 +188,18 @@ __global__ void __launch_bounds__(kVjpThreads) vjp_kernel(const TileArgs a, doub
     constexpr int NB = BetaTraits<ORDER>::N;
     constexpr int LDB = BetaTraits<ORDER>::LD;
     constexpr int NR = BetaTraits<ORDER>::NR;
+    constexpr bool ALL = ORDER == 5;   // stage every pencil of the tile at once, pipelined tiles
 
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int np = a.np, ldj = a.ldj, G = a.groups, jn = a.jn;
-    double* const tile = reinterpret_cast<double*>(smem_raw);     // [np][ldj] inputs
-    double* const gtile = tile + np * ldj;                         // [np][ldj] gradient
-    constexpr bool ALL = ORDER == 5;
+    double* const tile0 = reinterpret_cast<double*>(smem_raw);    // [np][ldj] inputs
+    double* const tile1 = tile0 + (ALL ? np * ldj : 0);            // second input buffer (WENO5)
+    double* const gtile = tile1 + np * ldj;                        // [np][ldj] gradient
     double* const cbuf = gtile + np * ldj;                         // [np or G][W][jn] contributions
     double* const beta_lds = cbuf + (ALL ? np : G) * W * jn;       // coefficient table
-    __shared__ long long s_in[kVjpMaxNP];
-    __shared__ long long s_out[kVjpMaxNP];
+    constexpr int NRING = ALL ? 3 : 1;
+    __shared__ long long s_in[NRING][kVjpMaxNP];
+    __shared__ long long s_out[NRING][kVjpMaxNP];
 
     const int tid = threadIdx.x, T = blockDim.x;
     const int ntile = a.tiles_per_var * a.nv;
@@ -233,8 +242,9 @@ __global__ void __launch_bounds__(kVjpThreads) vjp_kernel(const TileArgs a, doub
     // this lane's tile element s (the same for every tile): e = tid + s*T is
     // (pencil q, row J) with e = q*nJ + J for contiguous rows, e = J*np + q
     // otherwise (consecutive lanes read consecutive addresses); packed as
-    // (q << 16) | J, -1 for none.  Recomputed where used: held in registers
-    // for the whole launch it would push WENO7 past 256 VGPRs.
+    // (q << 16) | J, -1 for none.  WENO5 holds the table in registers for
+    // the whole launch; WENO7 recomputes it where used (held, it would push
+    // the kernel past 256 VGPRs).
     auto slot = [&](int s) -> int {
         const int e = tid + s * T;
         if (s >= a.slots || e >= np * nJ) return -1;
@@ -248,55 +258,77 @@ __global__ void __launch_bounds__(kVjpThreads) vjp_kernel(const TileArgs a, doub
         }
         return (q << 16) | J;
     };
-    for (int t = t_first; t < t_end; t += nbx) {
-        if (tid < np) {
-            const int iv = t / a.tiles_per_var;
-            const int P = (t - iv * a.tiles_per_var) * np + tid;
-            if (P < npen) {
-                const int pa = P % a.na, pb = P / a.na;
-                s_in[tid] = a.src_off + pa * a.s_a + pb * a.s_b + iv * a.s_i + lo * a.s_j;
-                const int k = a.a_is_k ? pa : pb;
-                const int l = a.a_is_k ? pb : pa;
-                s_out[tid] = k * a.o_k + l * a.o_l + iv * a.o_i;
-            } else {
-                s_in[tid] = -1;
-                s_out[tid] = -1;
-            }
+    // addresses of tile t's pencils into ring entry k (threads < np)
+    auto plan = [&](int t, int k) {
+        const int iv = t / a.tiles_per_var;
+        const int P = (t - iv * a.tiles_per_var) * np + tid;
+        if (P < npen) {
+            const int pa = P % a.na, pb = P / a.na;
+            s_in[k][tid] = a.src_off + pa * a.s_a + pb * a.s_b + iv * a.s_i + lo * a.s_j;
+            const int kk = a.a_is_k ? pa : pb;
+            const int l = a.a_is_k ? pb : pa;
+            s_out[k][tid] = kk * a.o_k + l * a.o_l + iv * a.o_i;
+        } else {
+            s_in[k][tid] = -1;
+            s_out[k][tid] = -1;
         }
-        __syncthreads();
-        // load the input tile: every slot's load is issued before the first
-        // LDS store, so a lane waits for one memory latency per tile
-        {
-            double r[kVjpSlots];
-            int sl[kVjpSlots];
+    };
+    // the whole slot table of lane `me` with one division: consecutive
+    // slots are T elements apart.  `me` is the thread index passed through
+    // an empty asm statement, so that the table is rebuilt where it is used
+    // rather than hoisted out of the tile loop into registers held for the
+    // whole launch.
+    auto slot_table = [&](int me, int* sl) {
+        const bool rows = a.s_j == 1;           // e = q*nJ + J, else e = J*np + q
+        const int div = rows ? nJ : np, lim = rows ? np : nJ;
+        const int step_hi = T / div, step_lo = T - step_hi * div;
+        int hi = me / div, lo = me - hi * div;
 #pragma unroll
-            for (int s = 0; s < kVjpSlots; ++s) {
-                r[s] = 0.0;
-                sl[s] = slot(s);
-                if (sl[s] >= 0) {
-                    const long long b = s_in[sl[s] >> 16];
-                    if (b >= 0) r[s] = src[b + static_cast<long long>(sl[s] & 0xffff) * a.s_j];
-                }
+        for (int s = 0; s < kVjpSlots; ++s) {
+            sl[s] = (s < a.slots && hi < lim) ? (rows ? (hi << 16) | lo : (lo << 16) | hi) : -1;
+            hi += step_hi;
+            lo += step_lo;
+            if (lo >= div) {
+                lo -= div;
+                ++hi;
             }
+        }
+    };
+    // WENO5 helpers over a slot table: every slot's load is issued before
+    // the first use, so a lane waits for one memory latency per tile array
+    auto load_slots = [&](const int* sl, const double* arr, const long long* sin, double* r) {
 #pragma unroll
-            for (int s = 0; s < kVjpSlots; ++s)
-                if (sl[s] >= 0) tile[(sl[s] >> 16) * ldj + (sl[s] & 0xffff)] = r[s];
-        }
-        // WENO5: the first pencil's cotangents (each cell fetches the next
-        // pencil's while it computes, so their latency overlaps the
-        // arithmetic; WENO7 has no registers to spare and loads them per cell)
-        double gl_n = 0.0, gr_n = 0.0;
-        if (ALL && core) {
-            const long long ob = s_out[gq * a.npg];
-            if (ob >= 0) {
-                gl_n = gL[ob + j];
-                gr_n = gR[ob + j];
+        for (int s = 0; s < kVjpSlots; ++s) {
+            r[s] = 0.0;
+            if (sl[s] >= 0) {
+                const long long b = sin[sl[s] >> 16];
+                if (b >= 0) r[s] = arr[b + static_cast<long long>(sl[s] & 0xffff) * a.s_j];
             }
         }
-        __syncthreads();
-        // contributions of the outputs, one barrier, the sums per input
-        // position, one barrier: WENO5 stages every pencil of the tile at
-        // once, WENO7 (register-bound) one pencil per group at a time
+    };
+    auto put_slots = [&](const int* sl, double* dst, const double* r) {
+#pragma unroll
+        for (int s = 0; s < kVjpSlots; ++s)
+            if (sl[s] >= 0) dst[(sl[s] >> 16) * ldj + (sl[s] & 0xffff)] = r[s];
+    };
+    // gv += gtile over the tile (each input element belongs to one pencil,
+    // so no other lane or workgroup touches it)
+    auto store_grad = [&](const int* sl, const long long* sin, const double* g) {
+#pragma unroll
+        for (int s = 0; s < kVjpSlots; ++s) {
+            if (sl[s] >= 0) {
+                const int q = sl[s] >> 16, J = sl[s] & 0xffff;
+                const long long b = sin[q];
+                if (b >= 0) grad[b + static_cast<long long>(J) * a.s_j] = g[s] + gtile[q * ldj + J];
+            }
+        }
+    };
+    // contributions of the outputs, one barrier, the sums per input position
+    // into gtile: WENO5 stages every pencil of the tile at once, WENO7
+    // (register-bound) one pencil per group at a time with a barrier after
+    // each sum.  WENO5 fetches each pencil's cotangents one pencil ahead
+    // (gl_n, gr_n hold the first pencil's on entry).
+    auto reconstruct = [&](const double* tile, const long long* sout, double gl_n, double gr_n) {
         for (int q0 = 0; q0 < a.npg; q0 += (ALL ? a.npg : 1)) {
             const int q1 = ALL ? a.npg : q0 + 1;
 #pragma unroll 1
@@ -304,12 +336,12 @@ __global__ void __launch_bounds__(kVjpThreads) vjp_kernel(const TileArgs a, doub
                 const int q = gq * a.npg + qq;
                 double* const cb = cbuf + (ALL ? q : gq) * W * jn;
                 if (core) {
-                    const long long ob = s_out[q];
+                    const long long ob = sout[q];
                     double gl = gl_n, gr = gr_n;
                     if constexpr (ALL) {
                         gl_n = gr_n = 0.0;
                         if (qq + 1 < a.npg) {
-                            const long long obn = s_out[q + 1];
+                            const long long obn = sout[q + 1];
                             if (obn >= 0) {
                                 gl_n = gL[obn + j];
                                 gr_n = gR[obn + j];
@@ -353,31 +385,108 @@ __global__ void __launch_bounds__(kVjpThreads) vjp_kernel(const TileArgs a, doub
             }
             __syncthreads();
         }
-        // accumulate the tile's gradient with the coalesced load mapping: all
-        // reads of gv first, then the sums and stores (each input element
-        // belongs to one pencil, so no other lane or workgroup touches it)
-        {
-            double g[kVjpSlots];
-            int sl[kVjpSlots];
-#pragma unroll
-            for (int s = 0; s < kVjpSlots; ++s) {
-                g[s] = 0.0;
-                sl[s] = slot(s);
-                if (sl[s] >= 0) {
-                    const long long b = s_in[sl[s] >> 16];
-                    if (b >= 0) g[s] = grad[b + static_cast<long long>(sl[s] & 0xffff) * a.s_j];
-                }
-            }
-#pragma unroll
-            for (int s = 0; s < kVjpSlots; ++s) {
-                if (sl[s] >= 0) {
-                    const int q = sl[s] >> 16, J = sl[s] & 0xffff;
-                    const long long b = s_in[q];
-                    if (b >= 0) grad[b + static_cast<long long>(J) * a.s_j] = g[s] + gtile[q * ldj + J];
-                }
+    };
+    auto first_cot = [&](const long long* sout, double& gl, double& gr) {
+        gl = gr = 0.0;
+        if (ALL && core) {
+            const long long ob = sout[gq * a.npg];
+            if (ob >= 0) {
+                gl = gL[ob + j];
+                gr = gR[ob + j];
             }
         }
+    };
+
+    if constexpr (ALL) {
+        if (tid < np) {
+            plan(t_first, 0);
+            if (t_first + nbx < t_end) plan(t_first + nbx, 1);
+        }
         __syncthreads();
+        {
+            double r[kVjpSlots];
+            int sl[kVjpSlots];
+            slot_table(tid, sl);
+            load_slots(sl, src, s_in[0], r);
+            put_slots(sl, tile0, r);
+        }
+        __syncthreads();
+        int k = 0, b = 0;
+        for (int t = t_first; t < t_end; t += nbx) {
+            const int k1 = k == 2 ? 0 : k + 1, k2 = k1 == 2 ? 0 : k1 + 1;
+            const bool more = t + nbx < t_end;
+            double* const cur = b ? tile1 : tile0;
+            double* const nxt = b ? tile0 : tile1;
+            double g[kVjpSlots], r[kVjpSlots];
+            {
+                int me = tid, sl[kVjpSlots];
+                asm volatile("" : "+v"(me));
+                slot_table(me, sl);
+                load_slots(sl, grad, s_in[k], g);
+                if (more) load_slots(sl, src, s_in[k1], r);
+            }
+            double gl0, gr0;
+            first_cot(s_out[k], gl0, gr0);
+            reconstruct(cur, s_out[k], gl0, gr0);
+            // tile t's gradient out, tile t+1's input into the other buffer,
+            // tile t+2's addresses into the ring entry tile t-1 used
+            {
+                int me = tid, sl[kVjpSlots];
+                asm volatile("" : "+v"(me));
+                slot_table(me, sl);
+                store_grad(sl, s_in[k], g);
+                if (more) put_slots(sl, nxt, r);
+            }
+            if (tid < np && t + 2 * nbx < t_end) plan(t + 2 * nbx, k2);
+            __syncthreads();
+            k = k1;
+            b ^= 1;
+        }
+    } else {
+        for (int t = t_first; t < t_end; t += nbx) {
+            if (tid < np) plan(t, 0);
+            __syncthreads();
+            {
+                double r[kVjpSlots];
+                int sl[kVjpSlots];
+#pragma unroll
+                for (int s = 0; s < kVjpSlots; ++s) {
+                    r[s] = 0.0;
+                    sl[s] = slot(s);
+                    if (sl[s] >= 0) {
+                        const long long b = s_in[0][sl[s] >> 16];
+                        if (b >= 0) r[s] = src[b + static_cast<long long>(sl[s] & 0xffff) * a.s_j];
+                    }
+                }
+#pragma unroll
+                for (int s = 0; s < kVjpSlots; ++s)
+                    if (sl[s] >= 0) tile0[(sl[s] >> 16) * ldj + (sl[s] & 0xffff)] = r[s];
+            }
+            __syncthreads();
+            reconstruct(tile0, s_out[0], 0.0, 0.0);
+            {
+                double g[kVjpSlots];
+                int sl[kVjpSlots];
+#pragma unroll
+                for (int s = 0; s < kVjpSlots; ++s) {
+                    g[s] = 0.0;
+                    sl[s] = slot(s);
+                    if (sl[s] >= 0) {
+                        const long long b = s_in[0][sl[s] >> 16];
+                        if (b >= 0) g[s] = grad[b + static_cast<long long>(sl[s] & 0xffff) * a.s_j];
+                    }
+                }
+#pragma unroll
+                for (int s = 0; s < kVjpSlots; ++s) {
+                    if (sl[s] >= 0) {
+                        const int q = sl[s] >> 16, J = sl[s] & 0xffff;
+                        const long long b = s_in[0][q];
+                        if (b >= 0) grad[b + static_cast<long long>(J) * a.s_j] = g[s] + gtile[q * ldj + J];
+                    }
+                }
+            }
+            __syncthreads();
+        }
     }
 }
 
@@ -385,7 +494,7 @@ hipError_t launch_vjp(TileArgs a, int order, double* grad, hipStream_t st)
 {
     // tile elements are packed as (pencil << 16) | row
     if (a.ldj > 0xffff || a.np > kVjpMaxNP || a.slots > kVjpSlots) return hipErrorInvalidConfiguration;
-    const size_t lds = (2 * static_cast<size_t>(a.np) * a.ldj +
+    const size_t lds = ((order == 5 ? 3 : 2) * static_cast<size_t>(a.np) * a.ldj +
                         static_cast<size_t>(order == 5 ? a.np : a.groups) * order * a.jn +
                         static_cast<size_t>(order == 5 ? BetaTraits<5>::LD : BetaTraits<7>::LD) * a.jn) *
                        sizeof(double);
