@@ -48,6 +48,7 @@ def lib():
             "oracle_weno7": [G, i, vp, vp, vp, vp, vp, vp, vp, vp],
             "oracle_mp": [G, i, vp, vp, vp],
             "oracle_weno_vjp": [G, i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp],
+            "oracle_set_teno_ratio_out": [vp],
         }
         for name, args in protos.items():
             f = getattr(_lib, name)
@@ -120,6 +121,27 @@ def weno(g: OGrid, order: int, d: int, v_rs: np.ndarray, coeffs, vL: np.ndarray,
     fn = {3: lib().oracle_weno3, 5: lib().oracle_weno5, 7: lib().oracle_weno7}[order]
     fn(C.byref(g), d, _p(np.asfortranarray(v_rs)), *[_p(c) for c in cs], _p(vL), _p(vR))
     return vL, vR
+
+
+def teno_ratios(g: OGrid, order: int, d: int, v_rs: np.ndarray, coeffs):
+    """(ratios, bases) of the TENO branch of the order-5/7 kernel: the cut-off
+    ratios omega_q = alpha_q / sum(alpha) it compares with teno_CT and the
+    bases x_q = 1 + tau/beta_q of the powers alpha_q = x_q^6
+    (weno_standalone.f90:580-590, :1490-1500), each of shape
+    (ns, nk, nl, v, 3 or 4); zero outside the core."""
+    assert g.teno and not g.wenojs and order in (5, 7)
+    n = 3 if order == 5 else 4
+    ns, nk, nl, nv = v_rs.shape[0] - 6, v_rs.shape[1], v_rs.shape[2], v_rs.shape[3]
+    flat = np.zeros(2 * n * ns * nk * nl * nv)
+    z = np.zeros((ns, nk, nl, nv), order="F")
+    lib().oracle_set_teno_ratio_out(_p(flat))
+    try:
+        weno(g, order, d, v_rs, coeffs, z, z)
+    finally:
+        lib().oracle_set_teno_ratio_out(None)
+    # for output element o: ratios at 2n*o + q, bases at 2n*o + n + q
+    both = flat.reshape((2 * n, ns, nk, nl, nv), order="F").transpose(1, 2, 3, 4, 0)
+    return both[..., :n], both[..., n:]
 
 
 def mp(g: OGrid, d: int, v_rs: np.ndarray, vL: np.ndarray, vR: np.ndarray):

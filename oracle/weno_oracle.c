@@ -242,6 +242,15 @@ void oracle_weno3(const oracle_grid* g, int dir, const double* v_rs,
                 }
 }
 
+/* Test hook: when set, the TENO branches of oracle_weno5 / oracle_weno7 store
+ * each cell's cut-off ratios omega_q and the bases x_q = 1 + tau/beta_q of
+ * the powers (N = 3 or 4 stencils; for output element o, omega_q at
+ * 2N*o + q and x_q at 2N*o + N + q), so that a test can place teno_CT
+ * exactly on a ratio the reference forms.  NULL (the default) stores
+ * nothing. */
+static double* teno_ratio_out = NULL;
+void oracle_set_teno_ratio_out(double* p) { teno_ratio_out = p; }
+
 /* weno_standalone.f90:477-669 (s_weno5_kernel; identical maths in :671-1397) */
 void oracle_weno5(const oracle_grid* g, int dir, const double* v_rs,
                   const double* polyL, const double* polyR, const double* dL,
@@ -302,6 +311,11 @@ void oracle_weno5(const oracle_grid* g, int dir, const double* v_rs,
                         alpha0 = pow(1.0 + tau / beta0, 6.0);
                         alpha1 = pow(1.0 + tau / beta1, 6.0);
                         alpha2 = pow(1.0 + tau / beta2, 6.0);
+                        if (teno_ratio_out) {
+                            teno_ratio_out[6 * o + 3] = 1.0 + tau / beta0;
+                            teno_ratio_out[6 * o + 4] = 1.0 + tau / beta1;
+                            teno_ratio_out[6 * o + 5] = 1.0 + tau / beta2;
+                        }
                         sum_alpha = alpha0 + alpha1 + alpha2;
                         omega0 = alpha0 / sum_alpha;
                         omega1 = alpha1 / sum_alpha;
@@ -309,6 +323,11 @@ void oracle_weno5(const oracle_grid* g, int dir, const double* v_rs,
                         delta0 = (omega0 < ct) ? 0.0 : 1.0;
                         delta1 = (omega1 < ct) ? 0.0 : 1.0;
                         delta2 = (omega2 < ct) ? 0.0 : 1.0;
+                        if (teno_ratio_out) {
+                            teno_ratio_out[6 * o + 0] = omega0;
+                            teno_ratio_out[6 * o + 1] = omega1;
+                            teno_ratio_out[6 * o + 2] = omega2;
+                        }
                         alpha0 = delta0 * dL[0 + 5 * jc];
                         alpha1 = delta1 * dL[1 + 5 * jc];
                         alpha2 = delta2 * dL[2 + 5 * jc];
@@ -410,6 +429,8 @@ void oracle_weno7(const oracle_grid* g, int dir, const double* v_rs,
                         /* :1490-1503 */
                         tau = fabs(beta_[4] - beta_[3]);
                         for (int q = 0; q < 5; ++q) alpha[q] = 1.0 + tau / beta_[q];
+                        if (teno_ratio_out)
+                            for (int q = 0; q < 4; ++q) teno_ratio_out[8 * o + 4 + q] = alpha[q];
                         /* (alpha**3._wp)**2._wp: the reference build evaluates
                          * the real power 3 as alpha*alpha*alpha (bitwise
                          * against oracle/_ref/teno) and the power 6 of
@@ -422,6 +443,7 @@ void oracle_weno7(const oracle_grid* g, int dir, const double* v_rs,
                         for (int q = 0; q < 5; ++q) omega[q] = alpha[q] / s;
                         for (int q = 0; q < 4; ++q) {
                             delta[q] = (omega[q] < ct) ? 0.0 : 1.0;
+                            if (teno_ratio_out) teno_ratio_out[8 * o + q] = omega[q];
                             alpha[q] = delta[q] * dL[q + 5 * jc];
                         }
                     }

@@ -65,6 +65,9 @@ void oracle_mp(const oracle_grid* g, int dir, const double* v_rs, double* vL, do
 /* A6: :1617-1620 */
 double oracle_checksum(const double* a, size_t n);
 
+/* test hook: per-cell TENO cut-off ratios of the next oracle_weno5/7 calls */
+void oracle_set_teno_ratio_out(double* p);
+
 /* Vector-Jacobian product of WENO5/7-JS (no reference counterpart; the
  * derivative of oracle_weno5 / oracle_weno7 with wenojs=T).
  * gv has the reshaped-input shape; it is overwritten. */

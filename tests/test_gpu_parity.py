@@ -3,15 +3,17 @@
 Tolerances, stated per kernel:
   reshape, order-1 copy, fused halo copy ........ bitwise
   WENO3/5/7-JS FP64, fused MP, standalone MP ..... 1e-13 relative (rel_err)
-  TENO FP64 ...................................... 1e-13 on cells whose
-      cut-off decision agrees; decisions may flip only where the oracle's
-      weight is within 1e-9 of teno_CT (none occur in these inputs)
+  TENO FP64 ...................................... 1e-13; the cut-off
+      decisions equal the reference's, also with teno_CT placed exactly on
+      a ratio the reference forms (test_teno_ct_on_a_reference_ratio)
   FP32 storage ................................... 1e-6 relative to the FP64 oracle
   BF16 state ..................................... 1e-6 relative to the FP64 oracle
       fed the same BF16-rounded inputs (the state's own rounding, 3.9e-3,
       is reported separately by the precision tests)
 """
 import ctypes as C
+import math
+from fractions import Fraction
 
 import numpy as np
 import pytest
@@ -144,6 +146,69 @@ def test_teno_vs_oracle(order):
         eL, eR = oracle.weno(og, order, d, oracle.reshape(og, d, f), cs, cL, cR)
         L, R = run_hip(g, order, d, vf, wm.SRC_FIELD, coef, wm.FUSE_COPY)
         assert rel_err(L, eL) <= 1e-13 and rel_err(R, eR) <= 1e-13
+
+
+@pytest.mark.parametrize("order", [5, 7])
+def test_teno_ct_on_a_reference_ratio(order):
+    """teno_CT set exactly to a cut-off ratio alpha_q/sum(alpha) the oracle
+    (the reference's arithmetic) forms at one cell: there the reference keeps
+    the stencil (ratio < CT is false), and a ratio formed with another
+    rounding would fall on either side.  The kernel must send that cell to
+    its exact arithmetic and agree with the oracle, at that cell to 1e-12
+    element-wise.  For WENO5 the oracle itself is checked to change that
+    cell when CT moves one ulp up, so a wrong decision could not pass.
+    TENO5's alpha_q = x_q^6 is the C library's pow in the reference and a
+    correctly rounded x^6 on the GPU; glibc's pow is not correctly rounded
+    for about 0.3 % of these bases (its own last-ulp errors, which no other
+    implementation reproduces), so CT is placed only on cells whose three
+    powers pow returns correctly rounded (checked with exact rationals)."""
+    g0, og0, f = _grid_and_field(seed=41 + order, wenojs=False, teno=True, teno_ct=1e-5)
+    d = 0
+    ns = fields.dir_extents(9, 12, 7, 6, d)[0]
+    cs = fields.harness_coefficients(ns - 6, fields.DIR_OFFSETS[d])
+    devs, coef = upload_coeffs(cs)
+    vf = DeviceArray.from_host(f)
+    v_rs = oracle.reshape(og0, d, f)
+    cL, cR = oracle.order1_copy(og0, d, f)
+    ratios, bases = oracle.teno_ratios(og0, order, d, v_rs, cs)
+    core = ratios[3:-3]
+    flat = core.reshape(-1)
+    # cells whose ratio is neither ~0 nor ~1 (WENO5) or any non-zero (WENO7)
+    cand = np.flatnonzero((flat > 1e-8) & (flat < 0.5)) if order == 5 else np.flatnonzero(flat > 0)
+    if order == 5:
+        xb = bases[3:-3].reshape(-1, 3)
+
+        def pow_is_cr(cell_stencil):
+            return all(math.pow(t, 6.0) == float(Fraction(t) ** 6) for t in xb[cell_stencil // 3])
+        cand = np.array([c for c in cand[np.linspace(0, cand.size - 1, 40).astype(int)] if pow_is_cr(c)])
+    assert cand.size >= 6
+    picks = cand[np.linspace(0, cand.size - 1, 6).astype(int)]
+    checked = 0
+    for idx in picks:
+        rho = float(flat[idx])
+        cell = np.unravel_index(idx, core.shape)[:4]
+        cell = (cell[0] + 3,) + tuple(cell[1:])
+        g, og, _ = _grid_and_field(seed=41 + order, wenojs=False, teno=True, teno_ct=rho)
+        eL, eR = oracle.weno(og, order, d, v_rs, cs, cL, cR)
+        if order == 5:
+            _, og_up, _ = _grid_and_field(seed=41 + order, wenojs=False, teno=True,
+                                          teno_ct=float(np.nextafter(rho, np.inf)))
+            uL, uR = oracle.weno(og_up, order, d, v_rs, cs, cL, cR)
+            if uL[cell] == eL[cell] and uR[cell] == eR[cell]:
+                continue            # this cell's decision does not move its output
+        L, R = run_hip(g, order, d, vf, wm.SRC_FIELD, coef, wm.FUSE_COPY)
+        print(f"teno{order} CT on ratio {rho!r} at cell {cell}", flush=True)
+        for name, got, ref in (("L", L, eL), ("R", R, eR)):
+            if rel_err(got, ref) > 1e-13:
+                err = np.abs(np.nan_to_num(got - ref))
+                w = np.unravel_index(np.argmax(err), err.shape)
+                print(f"  worst {name} at {w}: got {got[w]!r} ref {ref[w]!r} ratios {ratios[w].tolist()!r}",
+                      flush=True)
+        assert rel_err(L, eL) <= 1e-13 and rel_err(R, eR) <= 1e-13
+        for got, ref in ((L[cell], eL[cell]), (R[cell], eR[cell])):
+            assert ref != 0.0 and abs(got - ref) <= 1e-12 * abs(ref), (rho, cell, got, ref)
+        checked += 1
+    assert checked >= 3
 
 
 def test_no_weight_branch_gives_nan():

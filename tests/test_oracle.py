@@ -365,3 +365,30 @@ def test_vjp_complex_step_and_fd(order):
         hh = 1e-6
         fd = (J_of(x0 + hh) - J_of(x0 - hh)) / (2 * hh)
         assert abs(gv[mcell, 1, 1, 0] - fd) <= 1e-5 * max(1.0, abs(fd))
+
+
+@pytest.mark.parametrize("order", [5, 7])
+def test_teno_ratio_hook(order):
+    """oracle.teno_ratios reports the weights the cut-off compares: for TENO5
+    they are alpha_q / sum of the same alphas, so they sum to 1 to rounding;
+    for TENO7 the uncut alpha_4 takes the rest of the sum (below 1).  The
+    reported bases reproduce the ratios: alpha_q = pow(x_q, 6) (TENO5, the C
+    library's pow, as the oracle) or (x_q^3)^2 as products (TENO7)."""
+    og = oracle.grid(9, 12, 7, v=2, buff=6, wenojs=False, teno=True)
+    f = fields.seeded_field(9, 12, 7, 2, 6, seed=5)
+    ns = fields.dir_extents(9, 12, 7, 6, 1)[0]
+    cs = fields.harness_coefficients(ns - 6, fields.DIR_OFFSETS[1])
+    r, x = oracle.teno_ratios(og, order, 1, oracle.reshape(og, 1, f), cs)
+    core = r[3:-3]
+    assert np.all(x[3:-3] >= 1.0) and np.all(x[:3] == 0)   # x = 1 + tau/beta, beta > 0
+    assert np.all(r[:3] == 0) and np.all(r[-3:] == 0)
+    assert np.all(core >= 0)
+    total = core.sum(axis=-1)
+    if order == 5:
+        assert np.max(np.abs(total - 1.0)) <= 4e-16
+        import math
+        a = np.vectorize(lambda t: math.pow(t, 6.0))(x[3:-3])   # the C library's pow
+        s = a[..., 0] + a[..., 1] + a[..., 2]
+        assert np.array_equal(a / s[..., None], core)
+    else:
+        assert np.all(total <= 1.0)

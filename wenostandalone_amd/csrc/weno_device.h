@@ -239,7 +239,51 @@ __device__ __forceinline__ double masked_combine(const double (&a)[N], const C (
         S += a[r];
         num = fma(a[r], static_cast<double>(p[r]), num);
     }
-    return num / (S + extra);
+    const double D = S + extra;
+    return (D >= DBL_MIN && D <= DBL_MAX) ? fast_div(num, D) : num / D;
+}
+
+// 1/d to about one ulp: the hardware estimate and two Newton steps.
+__device__ __forceinline__ double rcp2(double d)
+{
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(r, fma(-d, r, 1.0), r);
+    return fma(r, fma(-d, r, 1.0), r);
+}
+
+// TENO cut-off decisions a_r / s < ct (weno_standalone.f90:584-590,
+// :1494-1500), screened.  The weights a_r = (1 + tau/beta_r)^6 enter the
+// result only through these decisions, so they are first formed with
+// reciprocal-based quotients and plain products: with every
+// x_r = 1 + tau/beta_r >= 1/2 (no cancellation) and a finite sum, each ratio
+// is within about 100 ulps (2.2e-14 relative) of the one the reference forms
+// with IEEE divisions and pow.  The decisions are taken from these ratios
+// when every ratio is more than 1e-12 ct away from ct; otherwise (or on any
+// NaN) the function returns false and the caller decides with the exact
+// arithmetic.  `extra` is added to the sum as is (TENO7's uncut a_4).
+template <int N>
+__device__ __forceinline__ bool teno_screen(const double (&b)[N], double tau, double extra, double ct,
+                                            double (&del)[N])
+{
+    double a[N], s = extra;
+    bool ok = true;
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+        const double x = fma(tau, rcp2(b[r]), 1.0);
+        ok = ok && (x >= 0.5);
+        const double t = x * x * x;
+        a[r] = t * t;
+        s += a[r];
+    }
+    ok = ok && (s <= DBL_MAX);
+    const double rs = rcp2(s), m = 1e-12 * fabs(ct);
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+        const double ratio = a[r] * rs;
+        ok = ok && (fabs(ratio - ct) > m);
+        del[r] = ratio < ct ? 0.0 : 1.0;
+    }
+    return ok;
 }
 
 // x**6 for TENO5's (1 + tau/beta)**6._wp (weno_standalone.f90:580-582), which
@@ -249,9 +293,12 @@ __device__ __forceinline__ double masked_combine(const double (&a)[N], const C (
 // relative error about 2^-100 before that rounding, so the result is x^6
 // correctly rounded (pow's result to within its own < 1 ulp) at about a tenth
 // of pow's instruction count.  When x^6 overflows the plain product (inf) is
-// returned, as pow does.
+// returned, as pow does.  Contraction is off: fused into the final sum, the
+// product p6 = p3*p3 would count its own rounding error twice (p3*p3 + e6
+// instead of p6 + e6).
 __device__ __forceinline__ double pow6(double x)
 {
+    _Pragma("clang fp contract(off)")
     const double p2 = x * x, e2 = fma(x, x, -p2);
     const double p3 = p2 * x, e3 = fma(p2, x, -p3) + e2 * x;
     const double p6 = p3 * p3, e6 = fma(p3, p3, -p6) + 2.0 * p3 * e3;
@@ -458,17 +505,24 @@ __device__ __forceinline__ void weno5(const C* v, const Cf<5, C>& f, const BT& B
             oL = wenoz_combine<3, 1, C>(bb, tau, dl, pL);
             oR = wenoz_combine<3, 1, C>(bb, tau, dr, pR);
         } else {
-            // :578-606, with IEEE divisions
+            // :578-606: decisions screened (teno_screen), else IEEE
+            // divisions and pow6 as the reference, no product fused into a
+            // sum
             const double tau = fabs(b2 - b0);
-            const double a0 = pow6(1.0 + tau / b0);
-            const double a1 = pow6(1.0 + tau / b1);
-            const double a2 = pow6(1.0 + tau / b2);
-            const double s = a0 + a1 + a2;
-            const double del0 = (a0 / s < ct) ? 0.0 : 1.0;
-            const double del1 = (a1 / s < ct) ? 0.0 : 1.0;
-            const double del2 = (a2 / s < ct) ? 0.0 : 1.0;
-            const double aL[3] = {del0 * B.dl(0), del1 * B.dl(1), del2 * B.dl(2)};
-            const double aR[3] = {del0 * B.dr(0), del1 * B.dr(1), del2 * B.dr(2)};
+            const double bb[3] = {b0, b1, b2};
+            double del[3];
+            if (!teno_screen<3>(bb, tau, 0.0, ct, del)) {
+                _Pragma("clang fp contract(off)")
+                const double a0 = pow6(1.0 + tau / b0);
+                const double a1 = pow6(1.0 + tau / b1);
+                const double a2 = pow6(1.0 + tau / b2);
+                const double s = a0 + a1 + a2;
+                del[0] = (a0 / s < ct) ? 0.0 : 1.0;
+                del[1] = (a1 / s < ct) ? 0.0 : 1.0;
+                del[2] = (a2 / s < ct) ? 0.0 : 1.0;
+            }
+            const double aL[3] = {del[0] * B.dl(0), del[1] * B.dl(1), del[2] * B.dl(2)};
+            const double aR[3] = {del[0] * B.dr(0), del[1] * B.dr(1), del[2] * B.dr(2)};
             oL = masked_combine<3, C>(aL, pL, 0.0);
             oR = masked_combine<3, C>(aR, pR, 0.0);
         }
@@ -514,27 +568,35 @@ __device__ __forceinline__ void weno7(const C* v, const Cf<7, C>& f, const BT& B
             // :1490-1503: beta(4) keeps weno_eps, alpha(4) is never cut.  The
             // real power 3 is the product x*x*x, as the reference build
             // evaluates it (the cut-off is discontinuous in these values).
+            // The decisions are screened (teno_screen); a_4, which also
+            // enters the result, is formed as the reference forms it.
             const double tau = fabs(eps - b[3]);
-            double a[5];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const double x = 1.0 + tau / b[q];
-                const double t = x * x * x;
-                a[q] = t * t;
-            }
+            double a4;
             {
+                _Pragma("clang fp contract(off)")
                 const double x = 1.0 + tau / eps;
                 const double t = x * x * x;
-                a[4] = t * t;
+                a4 = t * t;
             }
-            const double s = a[0] + a[1] + a[2] + a[3] + a[4];
             double del[4];
+            if (!teno_screen<4>(b, tau, a4, ct, del)) {
+                // as the reference rounds it: no product fused into the sum
+                _Pragma("clang fp contract(off)")
+                double a[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) del[q] = (a[q] / s < ct) ? 0.0 : 1.0;
+                for (int q = 0; q < 4; ++q) {
+                    const double x = 1.0 + tau / b[q];
+                    const double t = x * x * x;
+                    a[q] = t * t;
+                }
+                const double s = a[0] + a[1] + a[2] + a[3] + a4;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) del[q] = (a[q] / s < ct) ? 0.0 : 1.0;
+            }
             const double aL[4] = {del[0] * B.dl(0), del[1] * B.dl(1), del[2] * B.dl(2), del[3] * B.dl(3)};
             const double aR[4] = {del[0] * B.dr(0), del[1] * B.dr(1), del[2] * B.dr(2), del[3] * B.dr(3)};
-            oL = masked_combine<4, C>(aL, pL, a[4]);
-            oR = masked_combine<4, C>(aR, pR, a[4]);
+            oL = masked_combine<4, C>(aL, pL, a4);
+            oR = masked_combine<4, C>(aR, pR, a4);
         }
     }
 }
