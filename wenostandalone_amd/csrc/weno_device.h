@@ -305,31 +305,59 @@ __device__ __forceinline__ double pow6(double x)
     return fabs(p6) <= DBL_MAX ? p6 + e6 : p6;
 }
 
+__device__ __forceinline__ bool normal_mag(double x) { return fabs(x) >= DBL_MIN && fabs(x) <= DBL_MAX; }
+
 // Mapped WENO (weno_standalone.f90:433-437, :557-570, :1478-1482): the JS
 // weights w_r = a_r / sum(a), a_r = d_r / beta_r^2, mapped through
 //   g_r(w) = (d_r (1 + d_r - 3 w) + w^2) * (w / (d_r^2 + w (1 - 2 d_r)))
-// (Henrick, Aslam & Powers 2005), then normalised again.  IEEE divisions, as
-// the reference: the mapping is evaluated once per face and side.
+// (Henrick, Aslam & Powers 2005), then normalised again.  The quotients are
+// formed with reciprocals refined to about one ulp (rcp2; the reciprocals of
+// beta_r^2 are the same for both sides), which moves the face by a few ulps
+// against the reference's IEEE divisions; a lane whose denominators leave the
+// normal range uses IEEE divisions throughout.
 template <int N, typename C>
 __device__ __forceinline__ double mapped_combine(const double (&q)[N], const double (&d)[N], const C (&p)[N])
 {
     double a[N], s = 0.0;
+    bool ok = true;
 #pragma unroll
     for (int r = 0; r < N; ++r) {
-        a[r] = d[r] / q[r];
+        ok = ok && normal_mag(q[r]);
+        a[r] = d[r] * rcp2(q[r]);
         s += a[r];
     }
+    ok = ok && normal_mag(s);
+    if (ok) {
+        const double rs = rcp2(s);
 #pragma unroll
-    for (int r = 0; r < N; ++r) {
-        const double w = a[r] / s;
-        a[r] = (d[r] * (1.0 + d[r] - 3.0 * w) + w * w) * (w / (d[r] * d[r] + w * (1.0 - 2.0 * d[r])));
+        for (int r = 0; r < N; ++r) {
+            const double w = a[r] * rs;
+            const double den = d[r] * d[r] + w * (1.0 - 2.0 * d[r]);
+            ok = ok && normal_mag(den);
+            a[r] = (d[r] * (1.0 + d[r] - 3.0 * w) + w * w) * (w * rcp2(den));
+        }
+    }
+    if (!ok) {
+        s = 0.0;
+#pragma unroll
+        for (int r = 0; r < N; ++r) {
+            a[r] = d[r] / q[r];
+            s += a[r];
+        }
+#pragma unroll
+        for (int r = 0; r < N; ++r) {
+            const double w = a[r] / s;
+            a[r] = (d[r] * (1.0 + d[r] - 3.0 * w) + w * w) * (w / (d[r] * d[r] + w * (1.0 - 2.0 * d[r])));
+        }
     }
     return masked_combine<N, C>(a, p, 0.0);
 }
 
 // WENO-Z (weno_standalone.f90:439-441, :572-576, :1484-1488):
 // a_r = d_r (1 + (tau / beta_r)^Q), Q = 1 for WENO3/5 and wenoz_q = 2 for
-// WENO7 as the reference writes them.
+// WENO7 as the reference writes them.  IEEE divisions: the quotients are
+// shared by both sides, and a reciprocal-based form with its normal-range
+// fallback measured slower here (more registers than the kernel holds).
 template <int N, int Q, typename C>
 __device__ __forceinline__ double wenoz_combine(const double (&b)[N], double tau, const double (&d)[N],
                                                 const C (&p)[N])
