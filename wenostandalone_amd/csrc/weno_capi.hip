@@ -186,6 +186,7 @@ constexpr int kDefaultWeno7Threads = 256;
 constexpr int kDefaultWeno3Threads = 512;
 constexpr int kDefaultWeno5Threads = 448;
 constexpr int kDefaultMpThreads = 512;
+constexpr int kDefaultMappedWeno5Threads = 256;
 
 // Pencils per tile, j-chunking and LDS row stride for one launch.
 void plan_tiles(TileArgs& a, Launch& l, int np_override, int chunk_override, int threads_cap, int prefetch)
@@ -212,6 +213,10 @@ void plan_tiles(TileArgs& a, Launch& l, int np_override, int chunk_override, int
     // lane groups per block: best lane utilisation within 1024 threads
     int G = 1, T = 64 * ((jn + 63) / 64);
     double best = double(jn) / T;
+    // FP64 mapped WENO5 (heavier arithmetic per face): 256-thread tiles, two
+    // or more workgroups per CU (measured 1028 -> 984 us per 200^3 x 8 sweep)
+    if (threads_cap <= 0 && l.op == OP_WENO && l.order == 5 && l.mode == MODE_MAPPED && l.dtype == DT_F64)
+        threads_cap = kDefaultMappedWeno5Threads;
     if (threads_cap <= 0 && l.op == OP_WENO && l.order == 5) threads_cap = kDefaultWeno5Threads;
     // FP64 WENO7 with its coefficients in registers (~200-250 VGPRs): two
     // 256-thread workgroups per CU hold 8 waves where one 448-thread
