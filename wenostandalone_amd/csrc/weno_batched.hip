@@ -204,11 +204,11 @@ __device__ void batch_block(const BatchDesc& D, int item, double* tile, long lon
                 double oL, oR;
                 const double* vc = v + (R - RW);
                 if constexpr (ORDER == 3)
-                    weno3<MODE, double>(vc, f, B, a.eps, a.teno_ct, oL, oR);
+                    weno3<MODE, double, false>(vc, f, B, a.eps, a.teno_ct, oL, oR);
                 else if constexpr (ORDER == 5)
-                    weno5<MODE, double>(vc, f, B, a.eps, a.teno_ct, oL, oR);
+                    weno5<MODE, double, false>(vc, f, B, a.eps, a.teno_ct, oL, oR);
                 else
-                    weno7<MODE, double>(vc, f, B, a.eps, a.teno_ct, oL, oR);
+                    weno7<MODE, double, false>(vc, f, B, a.eps, a.teno_ct, oL, oR);
                 if constexpr (MP) {
                     if (do_mp) mp_limit<double>(v + (R - 2), oL, oR);
                 }
