@@ -454,8 +454,9 @@ def main():
         print(json.dumps(record_traffic(*args.record_traffic, m=args.m)))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the sweep would shard "
-              "by sub-domain with a buff_size halo exchange; see DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (the sweep would shard by "
+              "sub-domain, one GPU per sub-box, with a buff_size halo exchange between neighbours).",
+              file=sys.stderr)
         sys.exit(2)
 
     if args.families:
