@@ -160,3 +160,30 @@ def test_batched_branch(branch):
             sc = stencil_scale(v_rs, order // 2 if order != 3 else 1)
             L, R = got[b][d]
             assert rel_err_stencil(L, eL, sc) <= 1e-13 and rel_err_stencil(R, eR, sc) <= 1e-13, (b, d)
+
+
+@pytest.mark.parametrize("order", [5, 7])
+def test_teno_fp32_state(order):
+    """TENO with FP32 state: the general kernel takes the stencil differences
+    and candidate polynomials in FP32, beta, tau and the (screened) cut-off
+    decisions in FP64; against the oracle fed the same FP32-rounded state,
+    1e-6 of the stencil's scale.  The closest cut-off ratio of these inputs
+    lies 3.6e-4 (relative) from teno_CT, far beyond what FP32 differences
+    move it, so every decision matches.  TENO7's faces are ~1e-90 here
+    (its uncut alpha_4), below FP32's range: hence the stencil-scale bound."""
+    m, n, p, v = 9, 12, 7, 3
+    g = wm.make_grid(m, n, p, v=v, buff=6, wenojs=False, teno=True)
+    og = oracle.grid(m, n, p, v=v, buff=6, wenojs=False, teno=True)
+    f32 = fields.seeded_field(m, n, p, v, 6, seed=700 + order, recipe=2).astype(np.float32)
+    f64r = f32.astype(np.float64)
+    for d in range(3):
+        ns, nk, nl = fields.dir_extents(m, n, p, 6, d)
+        cs = fields.harness_coefficients(ns - 6, fields.DIR_OFFSETS[d])
+        devs, coef = _coef(cs)
+        v_rs = oracle.reshape(og, d, f64r)
+        sc = stencil_scale(v_rs, order // 2)
+        cL, cR = oracle.order1_copy(og, d, f64r)
+        eL, eR = oracle.weno(og, order, d, v_rs, cs, cL, cR)
+        L, R = _run(g, order, d, DeviceArray.from_host(f32), wm.SRC_FIELD, coef, wm.FUSE_COPY, (ns, nk, nl, v),
+                    dtype=wm.F32)
+        assert rel_err_stencil(L, eL, sc) <= 1e-6 and rel_err_stencil(R, eR, sc) <= 1e-6, d
