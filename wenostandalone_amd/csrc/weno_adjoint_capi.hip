@@ -74,10 +74,10 @@ extern "C" int weno_reconstruct_vjp(const weno_grid* g, int order, int dir, int 
     // LDS per workgroup: input tiles (two for WENO5, which loads the next
     // tile during the arithmetic) and the gradient tile, the contributions
     // of the outputs (every pencil of the tile for WENO5, one per group for
-    // WENO7), the coefficient table.  Workgroups of at most 4 waves run two
-    // per CU (78 KB each); larger ones run alone and may take 150 KB.
+    // WENO7; 2(W-1) zero rows each), the coefficient table.  Workgroups of at most 4 waves run two
+    // per CU (78 KB each); larger ones run alone and may take 158 KB.
     auto lds_of = [&](int np) {
-        return ((order == 5 ? 3 : 2) * size_t(np) * ldj + size_t(order == 5 ? np : G) * W * jn +
+        return ((order == 5 ? 3 : 2) * size_t(np) * ldj + size_t(order == 5 ? np : G) * W * (jn + 2 * (W - 1)) +
                 size_t(entries) * jn) *
                sizeof(double);
     };
@@ -85,10 +85,10 @@ extern "C" int weno_reconstruct_vjp(const weno_grid* g, int order, int dir, int 
         return lds_of(G * npg) <= cap && (size_t(G) * npg * ldj + T - 1) / T <= size_t(vjp_max_slots()) &&
                G * npg <= vjp_max_np();
     };
-    const size_t cap = T <= 256 ? 78 * 1024 : 150 * 1024;
+    const size_t cap = T <= 256 ? 78 * 1024 : 158 * 1024;   // + 1.5 KB of static LDS <= 160 KB
     int npg = std::max(1, (order == 5 ? 16 : 8) / G);
     while (npg > 1 && !fits(npg, cap)) --npg;
-    if (!fits(npg, cap) && !fits(npg, 150 * 1024))
+    if (!fits(npg, cap) && !fits(npg, 158 * 1024))
         return fail(WENO_E_INVALID, "adjoint tile does not fit LDS for sweep length %d", d.ns);
     a.groups = G;
     a.npg = npg;

@@ -195,8 +195,13 @@ __global__ void __launch_bounds__(kVjpThreads) vjp_kernel(const TileArgs a, doub
     double* const tile0 = reinterpret_cast<double*>(smem_raw);    // [np][ldj] inputs
     double* const tile1 = tile0 + (ALL ? np * ldj : 0);            // second input buffer (WENO5)
     double* const gtile = tile1 + np * ldj;                        // [np][ldj] gradient
-    double* const cbuf = gtile + np * ldj;                         // [np or G][W][jn] contributions
-    double* const beta_lds = cbuf + (ALL ? np : G) * W * jn;       // coefficient table
+    // contributions: per pencil (WENO5) or lane group (WENO7) a slot of
+    // jn + 4R rows of W doubles, output jj's W values in row jj + 2R, and 2R
+    // zero rows at either end, so that the sum for an input position reads
+    // its W entries without bounds checks
+    const int cbs = (jn + 4 * R) * W;
+    double* const cbuf = gtile + np * ldj;
+    double* const beta_lds = cbuf + (ALL ? np : G) * cbs;          // coefficient table
     constexpr int NRING = ALL ? 3 : 1;
     __shared__ long long s_in[NRING][kVjpMaxNP];
     __shared__ long long s_out[NRING][kVjpMaxNP];
@@ -232,6 +237,13 @@ __global__ void __launch_bounds__(kVjpThreads) vjp_kernel(const TileArgs a, doub
                 beta_lds[jj * LDB + NB + r] = a.c.dL[r + 5 * jc];
                 beta_lds[jj * LDB + NB + NR + r] = a.c.dR[r + 5 * jc];
             }
+        }
+    }
+    {
+        const int pad = 2 * R, per = 2 * pad * W, nslot = ALL ? np : G;
+        for (int e = tid; e < nslot * per; e += T) {
+            const int sl = e / per, rem = e - sl * per, r = rem / W, o = rem - r * W;
+            cbuf[sl * cbs + (r < pad ? r : jn + r) * W + o] = 0.0;
         }
     }
     const int npen = a.na * a.nb;
@@ -334,7 +346,7 @@ __global__ void __launch_bounds__(kVjpThreads) vjp_kernel(const TileArgs a, doub
 #pragma unroll 1
             for (int qq = q0; qq < q1; ++qq) {
                 const int q = gq * a.npg + qq;
-                double* const cb = cbuf + (ALL ? q : gq) * W * jn;
+                double* const cb = cbuf + (ALL ? q : gq) * cbs + (jj + 2 * R) * W;
                 if (core) {
                     const long long ob = sout[q];
                     double gl = gl_n, gr = gr_n;
@@ -362,23 +374,22 @@ __global__ void __launch_bounds__(kVjpThreads) vjp_kernel(const TileArgs a, doub
                         for (int o = 0; o < W; ++o) c[o] = 0.0;
                     }
 #pragma unroll
-                    for (int o = 0; o < W; ++o) cb[o * jn + jj] = c[o];
+                    for (int o = 0; o < W; ++o) cb[o] = c[o];
                 }
             }
             __syncthreads();
             if (grp) {
-                const int ncore = c1 - c0 + 1;
                 for (int qq = q0; qq < q1; ++qq) {
                     const int q = gq * a.npg + qq;
-                    const double* const cb = cbuf + (ALL ? q : gq) * W * jn;
+                    const double* const cb = cbuf + (ALL ? q : gq) * cbs;
                     for (int m = jj; m < nJ; m += jn) {
-                        // position m receives c[o] of output jj' = m - o
+                        // position m receives c[o] of output m - o, held in
+                        // row m - o + 2R: entry (W-1-o)(W-1) from p (zero
+                        // rows outside the outputs)
+                        const double* const pm = cb + m * W + (W - 1);
                         double sum = 0.0;
 #pragma unroll
-                        for (int o = 0; o < W; ++o) {
-                            const int js = m - o;
-                            if (js >= 0 && js < ncore) sum += cb[o * jn + js];
-                        }
+                        for (int o = 0; o < W; ++o) sum += pm[(W - 1 - o) * (W - 1)];
                         gtile[q * ldj + m] = sum;
                     }
                 }
@@ -495,7 +506,7 @@ hipError_t launch_vjp(TileArgs a, int order, double* grad, hipStream_t st)
     // tile elements are packed as (pencil << 16) | row
     if (a.ldj > 0xffff || a.np > kVjpMaxNP || a.slots > kVjpSlots) return hipErrorInvalidConfiguration;
     const size_t lds = ((order == 5 ? 3 : 2) * static_cast<size_t>(a.np) * a.ldj +
-                        static_cast<size_t>(order == 5 ? a.np : a.groups) * order * a.jn +
+                        static_cast<size_t>(order == 5 ? a.np : a.groups) * order * (a.jn + 2 * (order - 1)) +
                         static_cast<size_t>(order == 5 ? BetaTraits<5>::LD : BetaTraits<7>::LD) * a.jn) *
                        sizeof(double);
     const void* kern = order == 5 ? reinterpret_cast<const void*>(vjp_kernel<5>)
