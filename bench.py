@@ -182,7 +182,7 @@ def _time(h, stream, fn, reps=10, warm=2, warm_s=0.3):
     return ms.value / reps
 
 
-def suite(path):
+def suite(path, parts=("precision", "batched", "adjoint")):
     """Secondary measurements (written to `path` as JSON):
       precision  - WENO5/7 at 200^3x8 on the recipe-2 field (1e3 jumps and
                    exactly constant boxes): FP64 vs FP32 state vs BF16 state,
@@ -193,7 +193,8 @@ def suite(path):
                    oracle by the test suite);
       batched    - 64 blocks, sides U{16..128} per axis, v in {5,8,10},
                    WENO5/WENO7 mixed, x/y/z as one hipGraph after autotuning;
-      adjoint    - WENO5/7-JS at 128^3x8: forward x+y+z vs forward + VJP."""
+      adjoint    - WENO5/7-JS at 128^3x8: forward x+y+z vs forward + VJP.
+    `parts` selects which of the three run."""
     import wenostandalone_amd as wm
     from wenostandalone_amd import fields
     from wenostandalone_amd.device import DeviceArray
@@ -203,143 +204,147 @@ def suite(path):
     stream = wm.lib.weno_get_stream()
     h = hip()
     report = {}
+    b = 6   # buff_size of every section
 
-    # ---- precision forms ------------------------------------------------
-    m, v, b = 199, 8, 6
-    g = wm.make_grid(m, v=v, buff=b)
-    nx = m + 1 + 2 * b
-    f = fields.seeded_field(m, m, m, v, b, seed=3, recipe=2)
-    f32 = f.astype(np.float32)
-    u = f32.view(np.uint32)
-    bf = np.asfortranarray(((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16))
-    del u
-    srcs = {wm.F64: DeviceArray.from_host(f), wm.F32: DeviceArray.from_host(f32),
-            wm.BF16: DeviceArray.from_host(bf)}
-    del f, f32, bf
-    coefs = []
-    for d in range(3):
-        devs = [DeviceArray.from_host(c) for c in fields.harness_coefficients(nx - 6, fields.DIR_OFFSETS[d])]
-        coefs.append((devs, wm.Coeffs(*[x.ptr for x in devs])))
-    faces = {dt: [(DeviceArray((nx, nx, nx, v), np.float64 if dt == wm.F64 else np.float32),
-                   DeviceArray((nx, nx, nx, v), np.float64 if dt == wm.F64 else np.float32))
-                  for _ in range(3)] for dt in (wm.F64, wm.F32)}
-    prec = {}
-    for order in (5, 7):
-        ref = None
-        row = {}
-        forms = [(wm.F64, "fp64", -1), (wm.F32, "fp32", -1), (wm.BF16, "bf16", -1)]
-        if order == 5:
-            # the general kernel's form: stencil arithmetic in FP32
-            forms += [(wm.F32, "fp32_general_kernel", 8 | 2), (wm.BF16, "bf16_general_kernel", 8 | 2)]
-        for dt, name, variant in forms:
-            fc = faces[wm.F64 if dt == wm.F64 else wm.F32]
-
-            def sweep():
-                for d in range(3):
-                    wm.check(wm.lib.weno_reconstruct(C.byref(g), order, d, wm.SRC_FIELD, dt, srcs[dt].vp,
-                                                     C.byref(coefs[d][1]), fc[d][0].vp, fc[d][1].vp,
-                                                     wm.FUSE_COPY))
-            for d in range(3):
-                wm.check(wm.lib.weno_set_tile(order, dt, d, 0, 0, 0, variant))
-            try:
-                ms = _time(h, stream, sweep)
-            finally:
-                for d in range(3):
-                    wm.check(wm.lib.weno_set_tile(order, dt, d, 0, 0, 0, -1))
-            ent = {"ms_xyz": ms}
-            if dt == wm.F64:
-                ref = [(L.download(), R.download()) for L, R in fc]
-            else:
-                errs = {}
-                for d, ax in enumerate("xyz"):
-                    for side, k in (("L", 0), ("R", 1)):
-                        got = fc[d][k].download().astype(np.float64)
-                        r = ref[d][k]
-                        rel = np.abs(got - r) / np.maximum(np.abs(r), 1e-3 * np.abs(r).max())
-                        errs[f"{ax}{side}"] = {"max": float(rel.max()), "rms": float(np.sqrt(np.mean(rel ** 2)))}
-                        del got, rel
-                ent["rel_err_vs_fp64"] = errs
-            row[name] = ent
-        for name in list(row):
-            if name != "fp64":
-                row[f"speedup_{name}"] = row["fp64"]["ms_xyz"] / row[name]["ms_xyz"]
-        prec[f"weno{order}"] = row
-    report["precision"] = prec
-    for dt in faces:
-        for L, R in faces[dt]:
-            L.free()
-            R.free()
-    for s in srcs.values():
-        s.free()
-
-    # ---- batched blocks --------------------------------------------------
-    rng = np.random.default_rng(2024)
-    specs = [(int(rng.integers(16, 129)) - 1, int(rng.integers(16, 129)) - 1, int(rng.integers(16, 129)) - 1,
-              int(rng.choice([5, 8, 10])), int(rng.choice([5, 7]))) for _ in range(64)]
-    blocks = (wm.Block * 64)()
-    keep = []
-    nfaces = 0
-    for i, (bm, bn, bp, bv, order) in enumerate(specs):
-        fld = fields.seeded_field(bm, bn, bp, bv, b, seed=100 + i)
-        vf = DeviceArray.from_host(fld)
-        keep.append(vf)
-        blk = blocks[i]
-        blk.grid = wm.make_grid(bm, bn, bp, v=bv, buff=b)
-        blk.order = order
-        blk.v_vf = vf.ptr
+    if "precision" in parts:
+        # ---- precision forms ------------------------------------------------
+        m, v, b = 199, 8, 6
+        g = wm.make_grid(m, v=v, buff=b)
+        nx = m + 1 + 2 * b
+        f = fields.seeded_field(m, m, m, v, b, seed=3, recipe=2)
+        f32 = f.astype(np.float32)
+        u = f32.view(np.uint32)
+        bf = np.asfortranarray(((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16))
+        del u
+        srcs = {wm.F64: DeviceArray.from_host(f), wm.F32: DeviceArray.from_host(f32),
+                wm.BF16: DeviceArray.from_host(bf)}
+        del f, f32, bf
+        coefs = []
         for d in range(3):
-            ns, nk, nl = fields.dir_extents(bm, bn, bp, b, d)
-            devs = [DeviceArray.from_host(c) for c in fields.harness_coefficients(ns - 6, fields.DIR_OFFSETS[d])]
-            keep += devs
-            blk.coeffs[d] = wm.Coeffs(*[x.ptr for x in devs])
-            L, R = DeviceArray((ns, nk, nl, bv)), DeviceArray((ns, nk, nl, bv))
-            keep += [L, R]
-            blk.vL[d], blk.vR[d] = L.ptr, R.ptr
-            nfaces += 2 * ns * nk * nl * bv
-    best = (C.c_int * 3)()
-    wm.check(wm.lib.weno_batched_autotune(blocks, 64, wm.FUSE_COPY, best))
-    gh = C.c_void_p()
-    wm.check(wm.lib.weno_batched_graph_create(blocks, 64, wm.FUSE_COPY, C.byref(gh)))
-    ms = _time(h, stream, lambda: wm.check(wm.lib.weno_graph_launch(gh)))
-    wm.check(wm.lib.weno_graph_destroy(gh))
-    report["batched"] = {"blocks": 64, "sides": "U{16..128} per axis", "v_size": "{5,8,10}", "orders": "{5,7}",
-                         "face_values_per_xyz": nfaces, "ms_xyz_graph": ms,
-                         "face_values_per_s": nfaces / (ms * 1e-3),
-                         "autotuned": {"threads": best[0], "pencils": best[1]}}
-    for x in keep:
-        x.free()
-    del keep
+            devs = [DeviceArray.from_host(c) for c in fields.harness_coefficients(nx - 6, fields.DIR_OFFSETS[d])]
+            coefs.append((devs, wm.Coeffs(*[x.ptr for x in devs])))
+        faces = {dt: [(DeviceArray((nx, nx, nx, v), np.float64 if dt == wm.F64 else np.float32),
+                       DeviceArray((nx, nx, nx, v), np.float64 if dt == wm.F64 else np.float32))
+                      for _ in range(3)] for dt in (wm.F64, wm.F32)}
+        prec = {}
+        for order in (5, 7):
+            ref = None
+            row = {}
+            forms = [(wm.F64, "fp64", -1), (wm.F32, "fp32", -1), (wm.BF16, "bf16", -1)]
+            if order == 5:
+                # the general kernel's form: stencil arithmetic in FP32
+                forms += [(wm.F32, "fp32_general_kernel", 8 | 2), (wm.BF16, "bf16_general_kernel", 8 | 2)]
+            for dt, name, variant in forms:
+                fc = faces[wm.F64 if dt == wm.F64 else wm.F32]
 
-    # ---- adjoint -------------------------------------------------------------
-    m, v = 127, 8
-    g = wm.make_grid(m, v=v, buff=b)
-    nx = m + 1 + 2 * b
-    vf = DeviceArray.from_host(fields.seeded_field(m, m, m, v, b, seed=4))
-    gv = DeviceArray(vf.shape)
-    coefs = []
-    for d in range(3):
-        devs = [DeviceArray.from_host(c) for c in fields.harness_coefficients(nx - 6, fields.DIR_OFFSETS[d])]
-        coefs.append((devs, wm.Coeffs(*[x.ptr for x in devs])))
-    L3 = [DeviceArray((nx, nx, nx, v)) for _ in range(3)]
-    R3 = [DeviceArray((nx, nx, nx, v)) for _ in range(3)]
-    cot = np.asfortranarray(np.random.default_rng(1).standard_normal((nx, nx, nx, v)))
-    gL, gR = DeviceArray.from_host(cot), DeviceArray.from_host(cot[::-1].copy(order="F"))
-    del cot
-    adj = {}
-    for order in (5, 7):
-        def fwd():
-            for d in range(3):
-                wm.check(wm.lib.weno_reconstruct(C.byref(g), order, d, wm.SRC_FIELD, wm.F64, vf.vp,
-                                                 C.byref(coefs[d][1]), L3[d].vp, R3[d].vp, 0))
+                def sweep():
+                    for d in range(3):
+                        wm.check(wm.lib.weno_reconstruct(C.byref(g), order, d, wm.SRC_FIELD, dt, srcs[dt].vp,
+                                                         C.byref(coefs[d][1]), fc[d][0].vp, fc[d][1].vp,
+                                                         wm.FUSE_COPY))
+                for d in range(3):
+                    wm.check(wm.lib.weno_set_tile(order, dt, d, 0, 0, 0, variant))
+                try:
+                    ms = _time(h, stream, sweep)
+                finally:
+                    for d in range(3):
+                        wm.check(wm.lib.weno_set_tile(order, dt, d, 0, 0, 0, -1))
+                ent = {"ms_xyz": ms}
+                if dt == wm.F64:
+                    ref = [(L.download(), R.download()) for L, R in fc]
+                else:
+                    errs = {}
+                    for d, ax in enumerate("xyz"):
+                        for side, k in (("L", 0), ("R", 1)):
+                            got = fc[d][k].download().astype(np.float64)
+                            r = ref[d][k]
+                            rel = np.abs(got - r) / np.maximum(np.abs(r), 1e-3 * np.abs(r).max())
+                            errs[f"{ax}{side}"] = {"max": float(rel.max()), "rms": float(np.sqrt(np.mean(rel ** 2)))}
+                            del got, rel
+                    ent["rel_err_vs_fp64"] = errs
+                row[name] = ent
+            for name in list(row):
+                if name != "fp64":
+                    row[f"speedup_{name}"] = row["fp64"]["ms_xyz"] / row[name]["ms_xyz"]
+            prec[f"weno{order}"] = row
+        report["precision"] = prec
+        for dt in faces:
+            for L, R in faces[dt]:
+                L.free()
+                R.free()
+        for s in srcs.values():
+            s.free()
 
-        def bwd():
+    if "batched" in parts:
+        # ---- batched blocks --------------------------------------------------
+        rng = np.random.default_rng(2024)
+        specs = [(int(rng.integers(16, 129)) - 1, int(rng.integers(16, 129)) - 1, int(rng.integers(16, 129)) - 1,
+                  int(rng.choice([5, 8, 10])), int(rng.choice([5, 7]))) for _ in range(64)]
+        blocks = (wm.Block * 64)()
+        keep = []
+        nfaces = 0
+        for i, (bm, bn, bp, bv, order) in enumerate(specs):
+            fld = fields.seeded_field(bm, bn, bp, bv, b, seed=100 + i)
+            vf = DeviceArray.from_host(fld)
+            keep.append(vf)
+            blk = blocks[i]
+            blk.grid = wm.make_grid(bm, bn, bp, v=bv, buff=b)
+            blk.order = order
+            blk.v_vf = vf.ptr
             for d in range(3):
-                wm.check(wm.lib.weno_reconstruct_vjp(C.byref(g), order, d, wm.SRC_FIELD, vf.vp,
-                                                     C.byref(coefs[d][1]), gL.vp, gR.vp, gv.vp))
-        t_f = _time(h, stream, fwd)
-        t_b = _time(h, stream, bwd)
-        adj[f"weno{order}"] = {"fwd_ms_xyz": t_f, "vjp_ms_xyz": t_b, "fwd_plus_bwd_over_fwd": (t_f + t_b) / t_f}
-    report["adjoint"] = adj
+                ns, nk, nl = fields.dir_extents(bm, bn, bp, b, d)
+                devs = [DeviceArray.from_host(c) for c in fields.harness_coefficients(ns - 6, fields.DIR_OFFSETS[d])]
+                keep += devs
+                blk.coeffs[d] = wm.Coeffs(*[x.ptr for x in devs])
+                L, R = DeviceArray((ns, nk, nl, bv)), DeviceArray((ns, nk, nl, bv))
+                keep += [L, R]
+                blk.vL[d], blk.vR[d] = L.ptr, R.ptr
+                nfaces += 2 * ns * nk * nl * bv
+        best = (C.c_int * 3)()
+        wm.check(wm.lib.weno_batched_autotune(blocks, 64, wm.FUSE_COPY, best))
+        gh = C.c_void_p()
+        wm.check(wm.lib.weno_batched_graph_create(blocks, 64, wm.FUSE_COPY, C.byref(gh)))
+        ms = _time(h, stream, lambda: wm.check(wm.lib.weno_graph_launch(gh)))
+        wm.check(wm.lib.weno_graph_destroy(gh))
+        report["batched"] = {"blocks": 64, "sides": "U{16..128} per axis", "v_size": "{5,8,10}", "orders": "{5,7}",
+                             "face_values_per_xyz": nfaces, "ms_xyz_graph": ms,
+                             "face_values_per_s": nfaces / (ms * 1e-3),
+                             "autotuned": {"threads": best[0], "pencils": best[1]}}
+        for x in keep:
+            x.free()
+        del keep
+
+    if "adjoint" in parts:
+        # ---- adjoint -------------------------------------------------------------
+        m, v = 127, 8
+        g = wm.make_grid(m, v=v, buff=b)
+        nx = m + 1 + 2 * b
+        vf = DeviceArray.from_host(fields.seeded_field(m, m, m, v, b, seed=4))
+        gv = DeviceArray(vf.shape)
+        coefs = []
+        for d in range(3):
+            devs = [DeviceArray.from_host(c) for c in fields.harness_coefficients(nx - 6, fields.DIR_OFFSETS[d])]
+            coefs.append((devs, wm.Coeffs(*[x.ptr for x in devs])))
+        L3 = [DeviceArray((nx, nx, nx, v)) for _ in range(3)]
+        R3 = [DeviceArray((nx, nx, nx, v)) for _ in range(3)]
+        cot = np.asfortranarray(np.random.default_rng(1).standard_normal((nx, nx, nx, v)))
+        gL, gR = DeviceArray.from_host(cot), DeviceArray.from_host(cot[::-1].copy(order="F"))
+        del cot
+        adj = {}
+        for order in (5, 7):
+            def fwd():
+                for d in range(3):
+                    wm.check(wm.lib.weno_reconstruct(C.byref(g), order, d, wm.SRC_FIELD, wm.F64, vf.vp,
+                                                     C.byref(coefs[d][1]), L3[d].vp, R3[d].vp, 0))
+
+            def bwd():
+                for d in range(3):
+                    wm.check(wm.lib.weno_reconstruct_vjp(C.byref(g), order, d, wm.SRC_FIELD, vf.vp,
+                                                         C.byref(coefs[d][1]), gL.vp, gR.vp, gv.vp))
+            t_f = _time(h, stream, fwd)
+            t_b = _time(h, stream, bwd)
+            adj[f"weno{order}"] = {"fwd_ms_xyz": t_f, "vjp_ms_xyz": t_b, "fwd_plus_bwd_over_fwd": (t_f + t_b) / t_f}
+        report["adjoint"] = adj
     wm.check(wm.lib.weno_set_async(0))
     with open(path, "w") as fh:
         json.dump(report, fh, indent=1)
@@ -443,6 +448,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--flags", type=int, default=0, help="extra weno_reconstruct flags")
     ap.add_argument("--suite", default=None, help="also run the secondary measurements into this JSON file")
+    ap.add_argument("--adjoint", default=None,
+                    help="run only the suite's adjoint measurement (WENO5/7 VJP at 128^3x8) into this JSON file")
     ap.add_argument("--families", default=None,
                     help="time every kernel family against its roofline into this JSON file, then exit")
     ap.add_argument("--copy-yardstick", action="store_true",
@@ -461,6 +468,9 @@ def main():
 
     if args.families:
         families(args.families, m=args.m)
+        return
+    if args.adjoint:
+        suite(args.adjoint, parts=("adjoint",))
         return
 
     import wenostandalone_amd as wm
