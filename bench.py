@@ -450,6 +450,10 @@ def main():
     ap.add_argument("--suite", default=None, help="also run the secondary measurements into this JSON file")
     ap.add_argument("--adjoint", default=None,
                     help="run only the suite's adjoint measurement (WENO5/7 VJP at 128^3x8) into this JSON file")
+    ap.add_argument("--batched", default=None,
+                    help="run only the suite's batched measurement (64 blocks, hipGraph) into this JSON file")
+    ap.add_argument("--precision", default=None,
+                    help="run only the suite's precision measurement (FP32/BF16 state) into this JSON file")
     ap.add_argument("--families", default=None,
                     help="time every kernel family against its roofline into this JSON file, then exit")
     ap.add_argument("--copy-yardstick", action="store_true",
@@ -469,9 +473,10 @@ def main():
     if args.families:
         families(args.families, m=args.m)
         return
-    if args.adjoint:
-        suite(args.adjoint, parts=("adjoint",))
-        return
+    for part in ("adjoint", "batched", "precision"):
+        if getattr(args, part):
+            suite(getattr(args, part), parts=(part,))
+            return
 
     import wenostandalone_amd as wm
     from wenostandalone_amd import fields
