@@ -86,7 +86,11 @@ extern "C" int weno_reconstruct_vjp(const weno_grid* g, int order, int dir, int 
                G * npg <= vjp_max_np();
     };
     const size_t cap = T <= 256 ? 78 * 1024 : 158 * 1024;   // + 1.5 KB of static LDS <= 160 KB
-    int npg = std::max(1, (order == 5 ? 16 : 8) / G);
+    // pencils per tile: WENO5 stages all of them at once (LDS-bound); WENO7
+    // stages one per group at a time, so its tiles can be larger, which
+    // spreads the per-tile loads and barriers over more cells (128^3 x 8:
+    // 6 -> 24 pencils, 1.90 -> 1.63 ms per x+y+z)
+    int npg = std::max(1, (order == 5 ? 16 : 24) / G);
     while (npg > 1 && !fits(npg, cap)) --npg;
     if (!fits(npg, cap) && !fits(npg, 158 * 1024))
         return fail(WENO_E_INVALID, "adjoint tile does not fit LDS for sweep length %d", d.ns);
