@@ -73,11 +73,11 @@ extern "C" int weno_reconstruct_vjp(const weno_grid* g, int order, int dir, int 
     const int ldj = nJ | 1;
     // LDS per workgroup: input tiles (two for WENO5, which loads the next
     // tile during the arithmetic) and the gradient tile, the contributions
-    // of the outputs (every pencil of the tile for WENO5, one per group for
-    // WENO7; 2(W-1) zero rows each), the coefficient table.  Workgroups of at most 4 waves run two
-    // per CU (78 KB each); larger ones run alone and may take 158 KB.
+    // of the outputs (one pencil per lane group, 2(W-1) zero rows each), the
+    // coefficient table.  Workgroups of at most 4 waves run two per CU (78 KB
+    // each); larger ones run alone and may take 158 KB.
     auto lds_of = [&](int np) {
-        return ((order == 5 ? 3 : 2) * size_t(np) * ldj + size_t(order == 5 ? np : G) * W * (jn + 2 * (W - 1)) +
+        return ((order == 5 ? 3 : 2) * size_t(np) * ldj + size_t(G) * W * (jn + 2 * (W - 1)) +
                 size_t(entries) * jn) *
                sizeof(double);
     };
@@ -86,11 +86,11 @@ extern "C" int weno_reconstruct_vjp(const weno_grid* g, int order, int dir, int 
                G * npg <= vjp_max_np();
     };
     const size_t cap = T <= 256 ? 78 * 1024 : 158 * 1024;   // + 1.5 KB of static LDS <= 160 KB
-    // pencils per tile: WENO5 stages all of them at once (LDS-bound); WENO7
-    // stages one per group at a time, so its tiles can be larger, which
-    // spreads the per-tile loads and barriers over more cells (128^3 x 8:
-    // 6 -> 24 pencils, 1.90 -> 1.63 ms per x+y+z)
-    int npg = std::max(1, (order == 5 ? 16 : 24) / G);
+    // pencils per tile: the contributions are staged one pencil per group at
+    // a time, so the tiles can be large, which spreads the per-tile loads and
+    // barriers over more cells (128^3 x 8: WENO5 15 -> 24 pencils, 1.24 ->
+    // 1.19 ms per x+y+z; WENO7 6 -> 24 pencils, 1.90 -> 1.63 ms)
+    int npg = std::max(1, 24 / G);
     while (npg > 1 && !fits(npg, cap)) --npg;
     if (!fits(npg, cap) && !fits(npg, 158 * 1024))
         return fail(WENO_E_INVALID, "adjoint tile does not fit LDS for sweep length %d", d.ns);

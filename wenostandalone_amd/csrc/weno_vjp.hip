@@ -188,20 +188,19 @@ __global__ void __launch_bounds__(kVjpThreads) vjp_kernel(const TileArgs a, doub
     constexpr int NB = BetaTraits<ORDER>::N;
     constexpr int LDB = BetaTraits<ORDER>::LD;
     constexpr int NR = BetaTraits<ORDER>::NR;
-    constexpr bool ALL = ORDER == 5;   // stage every pencil of the tile at once, pipelined tiles
+    constexpr bool ALL = ORDER == 5;   // pipelined tiles, cotangents fetched ahead (registers)
 
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int np = a.np, ldj = a.ldj, G = a.groups, jn = a.jn;
     double* const tile0 = reinterpret_cast<double*>(smem_raw);    // [np][ldj] inputs
     double* const tile1 = tile0 + (ALL ? np * ldj : 0);            // second input buffer (WENO5)
     double* const gtile = tile1 + np * ldj;                        // [np][ldj] gradient
-    // contributions: per pencil (WENO5) or lane group (WENO7) a slot of
-    // jn + 4R rows of W doubles, output jj's W values in row jj + 2R, and 2R
+    // contributions: per lane group a slot of jn + 4R rows of W doubles, output jj's W values in row jj + 2R, and 2R
     // zero rows at either end, so that the sum for an input position reads
     // its W entries without bounds checks
     const int cbs = (jn + 4 * R) * W;
     double* const cbuf = gtile + np * ldj;
-    double* const beta_lds = cbuf + (ALL ? np : G) * cbs;          // coefficient table
+    double* const beta_lds = cbuf + G * cbs;                       // coefficient table
     constexpr int NRING = ALL ? 3 : 1;
     __shared__ long long s_in[NRING][kVjpMaxNP];
     __shared__ long long s_out[NRING][kVjpMaxNP];
@@ -240,7 +239,7 @@ __global__ void __launch_bounds__(kVjpThreads) vjp_kernel(const TileArgs a, doub
         }
     }
     {
-        const int pad = 2 * R, per = 2 * pad * W, nslot = ALL ? np : G;
+        const int pad = 2 * R, per = 2 * pad * W, nslot = G;
         for (int e = tid; e < nslot * per; e += T) {
             const int sl = e / per, rem = e - sl * per, r = rem / W, o = rem - r * W;
             cbuf[sl * cbs + (r < pad ? r : jn + r) * W + o] = 0.0;
@@ -335,18 +334,17 @@ __global__ void __launch_bounds__(kVjpThreads) vjp_kernel(const TileArgs a, doub
             }
         }
     };
-    // contributions of the outputs, one barrier, the sums per input position
-    // into gtile: WENO5 stages every pencil of the tile at once, WENO7
-    // (register-bound) one pencil per group at a time with a barrier after
-    // each sum.  WENO5 fetches each pencil's cotangents one pencil ahead
-    // (gl_n, gr_n hold the first pencil's on entry).
+    // one pencil per lane group at a time: the contributions of its outputs,
+    // one barrier, the sums per input position into gtile, one barrier.  WENO5
+    // fetches each pencil's cotangents one pencil ahead (gl_n, gr_n hold the
+    // first pencil's on entry); WENO7 (register-bound) loads them per cell.
     auto reconstruct = [&](const double* tile, const long long* sout, double gl_n, double gr_n) {
-        for (int q0 = 0; q0 < a.npg; q0 += (ALL ? a.npg : 1)) {
-            const int q1 = ALL ? a.npg : q0 + 1;
+        for (int q0 = 0; q0 < a.npg; ++q0) {
+            const int q1 = q0 + 1;
 #pragma unroll 1
             for (int qq = q0; qq < q1; ++qq) {
                 const int q = gq * a.npg + qq;
-                double* const cb = cbuf + (ALL ? q : gq) * cbs + (jj + 2 * R) * W;
+                double* const cb = cbuf + gq * cbs + (jj + 2 * R) * W;
                 if (core) {
                     const long long ob = sout[q];
                     double gl = gl_n, gr = gr_n;
@@ -381,7 +379,7 @@ __global__ void __launch_bounds__(kVjpThreads) vjp_kernel(const TileArgs a, doub
             if (grp) {
                 for (int qq = q0; qq < q1; ++qq) {
                     const int q = gq * a.npg + qq;
-                    const double* const cb = cbuf + (ALL ? q : gq) * cbs;
+                    const double* const cb = cbuf + gq * cbs;
                     for (int m = jj; m < nJ; m += jn) {
                         // position m receives c[o] of output m - o, held in
                         // row m - o + 2R: entry (W-1-o)(W-1) from p (zero
@@ -506,7 +504,7 @@ hipError_t launch_vjp(TileArgs a, int order, double* grad, hipStream_t st)
     // tile elements are packed as (pencil << 16) | row
     if (a.ldj > 0xffff || a.np > kVjpMaxNP || a.slots > kVjpSlots) return hipErrorInvalidConfiguration;
     const size_t lds = ((order == 5 ? 3 : 2) * static_cast<size_t>(a.np) * a.ldj +
-                        static_cast<size_t>(order == 5 ? a.np : a.groups) * order * (a.jn + 2 * (order - 1)) +
+                        static_cast<size_t>(a.groups) * order * (a.jn + 2 * (order - 1)) +
                         static_cast<size_t>(order == 5 ? BetaTraits<5>::LD : BetaTraits<7>::LD) * a.jn) *
                        sizeof(double);
     const void* kern = order == 5 ? reinterpret_cast<const void*>(vjp_kernel<5>)
