@@ -9,9 +9,11 @@ straight from v_vf with direction-aware loads.  Inputs are resident in HBM
 before the timed region.  value = face values reconstructed per second.
 
 Prints one JSON line.  Also reports the dominant kernel's roofline figure
-(algorithmic bytes per launch / HIP-event launch time on the launch stream)
-and a CPU baseline (the reference's own OpenMP build on the host cores when
-present, else the oracle's C port).
+(algorithmic bytes per launch / HIP-event launch time on the launch stream),
+the metric's second half (the same step with the state stored in FP32 and in
+BF16: speedup over FP64 and element-wise error against the FP64 path, on the
+field with 1e3 jumps and exactly constant boxes) and a CPU baseline (the
+reference's own OpenMP build on every host core, else the oracle's C port).
 """
 from __future__ import annotations
 
@@ -103,11 +105,29 @@ def hip():
     return h
 
 
-def cpu_baseline(m: int, v: int, buff: int, budget_s: float = 30.0):
+def host_cores() -> int:
+    """Cores this process may run on (its CPU affinity set)."""
+    try:
+        return max(1, len(os.sched_getaffinity(0)))
+    except (AttributeError, OSError):
+        return max(1, os.cpu_count() or 1)
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            return [l.split(":", 1)[1].strip() for l in fh if l.startswith("model name")][0]
+    except (OSError, IndexError):
+        return "unknown CPU"
+
+
+def cpu_baseline(m: int, v: int, buff: int, budget_s: float = 60.0):
     """The reference's own CPU OpenMP build (oracle/_ref/weno_standalone,
-    compiled from the reference source) in weno5/kernel scope on the host
-    cores; falls back to the oracle's C port."""
-    cores = max(1, min(16, os.cpu_count() or 1))
+    compiled from the reference source) in weno5/kernel scope on every host
+    core this process may use, with the reference's published protocol of 5
+    warm-up and 20 timed iterations (results/amd_mp_roof_ex/log.txt:31-32);
+    falls back to the oracle's C port."""
+    cores = host_cores()
     faces = 3 * 2 * (m + 1 + 2 * buff - 6) * (m + 1 + 2 * buff) ** 2 * v
     ref = os.path.join(ROOT, "oracle", "_ref", "weno_standalone")
     if os.path.exists(ref):
@@ -118,18 +138,15 @@ def cpu_baseline(m: int, v: int, buff: int, budget_s: float = 30.0):
                          f" buff_size={buff}, weno_eps=1e-16, wenojs=T, teno_CT=1e-5\n/\n")
             env = dict(os.environ, OMP_TARGET_OFFLOAD="DISABLED", OMP_NUM_THREADS=str(cores),
                        OMP_PROC_BIND="close", WENO_KERNEL_MODE="weno5", WENO5_BENCH_SCOPE="kernel",
-                       WENO5_SPECIALIZED_COMBINED="1", WENO_WARMUP_ITERS="1", WENO_BENCH_ITERS="3")
+                       WENO5_SPECIALIZED_COMBINED="1", WENO_WARMUP_ITERS="5", WENO_BENCH_ITERS="20")
             out = subprocess.run([ref, nml], env=env, capture_output=True, text=True,
                                  timeout=budget_s * 4).stdout
         t = float(re.search(r"Kernel time per iter\s*:\s*([0-9.Ee+-]+)", out).group(1))
-        cpu = ""
-        try:
-            cpu = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
-        except Exception:
-            pass
         return {"value": faces / t, "unit": "face values/s", "cores": cores, "kind": "reference",
-                "sample": f"reference OpenMP build, weno5 kernel scope (x+y+z), {m + 1}^3x{v}, "
-                          f"1 warmup + 3 timed iterations, {t:.4f} s/iter; host {cpu}"}
+                "sample": f"reference OpenMP build (amdflang -fopenmp -O3, host OpenMP), weno5 kernel scope "
+                          f"(x+y+z), {m + 1}^3x{v}, OMP_NUM_THREADS={cores} OMP_PROC_BIND=close, "
+                          f"5 warm-up + 20 timed iterations, {t:.4f} s/iter; host {cpu_model()}, "
+                          f"{cores} cores in the affinity set"}
     import oracle
     from wenostandalone_amd import fields
     mm = 99   # bounded sample
@@ -147,7 +164,97 @@ def cpu_baseline(m: int, v: int, buff: int, budget_s: float = 30.0):
         n += time.perf_counter() - t1
     sfaces = 3 * 2 * (mm + 1 + 2 * buff - 6) * (mm + 1 + 2 * buff) ** 2 * v
     return {"value": sfaces / n, "unit": "face values/s", "cores": cores, "kind": "port",
-            "sample": f"oracle C port, WENO5 x+y+z at {mm + 1}^3x{v} ({n:.3f} s)"}
+            "sample": f"oracle C port, WENO5 x+y+z at {mm + 1}^3x{v} ({n:.3f} s); host {cpu_model()}"}
+
+
+def bf16_bits(f32: np.ndarray) -> np.ndarray:
+    """FP32 -> BF16 bit patterns, round to nearest even."""
+    u = f32.view(np.uint32)
+    return np.asfortranarray(((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16))
+
+
+def precision_leg(h, stream, m: int, v: int, buff: int, steps: int, warmup: int) -> dict:
+    """The metric's second half: WENO5-JS x+y+z at (m+1)^3 x v with the state
+    stored in FP64, FP32 and BF16 (FP32 faces for both), on the recipe-2 field
+    (1e3 jumps and exactly constant boxes), core-range outputs as in the
+    headline step.  Each form is timed over `steps` steps with HIP events on
+    the library stream after `warmup` steps; the FP32 / BF16 faces are
+    compared element-wise with the FP64 faces (the FP64 path is held to 1e-13
+    of the oracle by the test suite): relative error |a-b|/|b| over every core
+    face of the six outputs (values of this field are >= 0.1 in magnitude)."""
+    import wenostandalone_amd as wm
+    from wenostandalone_amd import fields
+    from wenostandalone_amd.device import DeviceArray
+
+    g = wm.make_grid(m, v=v, buff=buff)
+    nx = m + 1 + 2 * buff
+    f = fields.seeded_field(m, m, m, v, buff, seed=3, recipe=2)
+    f32 = f.astype(np.float32)
+    srcs = {wm.F64: DeviceArray.from_host(f), wm.F32: DeviceArray.from_host(f32),
+            wm.BF16: DeviceArray.from_host(bf16_bits(f32))}
+    del f, f32
+    coefs = []
+    for d in range(3):
+        devs = [DeviceArray.from_host(c) for c in fields.harness_coefficients(nx - 6, fields.DIR_OFFSETS[d])]
+        coefs.append((devs, wm.Coeffs(*[x.ptr for x in devs])))
+    faces = {}
+    for dt in (wm.F64, wm.F32, wm.BF16):
+        odt = np.float64 if dt == wm.F64 else np.float32
+        faces[dt] = [(DeviceArray((nx, nx, nx, v), odt), DeviceArray((nx, nx, nx, v), odt)) for _ in range(3)]
+        for d in range(3):
+            wm.check(wm.lib.weno_order1_copy(C.byref(g), d, dt, srcs[dt].vp, faces[dt][d][0].vp,
+                                             faces[dt][d][1].vp))
+
+    def step(dt):
+        for d in range(3):
+            wm.check(wm.lib.weno_reconstruct(C.byref(g), 5, d, wm.SRC_FIELD, dt, srcs[dt].vp, C.byref(coefs[d][1]),
+                                             faces[dt][d][0].vp, faces[dt][d][1].vp, 0))
+
+    wm.check(wm.lib.weno_set_async(1))
+    ms = {}
+    a, b = C.c_void_p(), C.c_void_p()
+    h.hipEventCreate(C.byref(a))
+    h.hipEventCreate(C.byref(b))
+    for dt in (wm.F64, wm.F32, wm.BF16):
+        for _ in range(warmup):
+            step(dt)
+        wm.check(wm.lib.weno_sync())
+        h.hipEventRecord(a, stream)
+        for _ in range(steps):
+            step(dt)
+        h.hipEventRecord(b, stream)
+        wm.check(wm.lib.weno_sync())
+        t = C.c_float()
+        h.hipEventElapsedTime(C.byref(t), a, b)
+        ms[dt] = t.value / steps
+    wm.check(wm.lib.weno_set_async(0))
+    core = slice(3, nx - 3)
+    err = {}
+    for dt, name in ((wm.F32, "fp32"), (wm.BF16, "bf16")):
+        mx, ss, n = 0.0, 0.0, 0
+        for d in range(3):
+            for k in range(2):
+                ref = faces[wm.F64][d][k].download()[core]
+                got = faces[dt][d][k].download()[core].astype(np.float64)
+                rel = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)
+                del got, ref
+                mx = max(mx, float(rel.max()))
+                ss += float(np.sum(rel * rel))
+                n += rel.size
+                del rel
+        err[name] = (mx, float(np.sqrt(ss / n)))
+    for dt in faces:
+        for L, R in faces[dt]:
+            L.free()
+            R.free()
+    for s in srcs.values():
+        s.free()
+    return {"fp32_speedup": ms[wm.F64] / ms[wm.F32], "fp32_max_rel_err": err["fp32"][0],
+            "fp32_rms_rel_err": err["fp32"][1], "bf16_speedup": ms[wm.F64] / ms[wm.BF16],
+            "bf16_max_rel_err": err["bf16"][0], "bf16_rms_rel_err": err["bf16"][1],
+            "ms_per_step": {"fp64": ms[wm.F64], "fp32": ms[wm.F32], "bf16": ms[wm.BF16]},
+            "field": "recipe 2 (sine modes + step + noise, 1e3 jump, exactly constant boxes), seed 3",
+            "error_reference": "FP64 path on the FP64 field, core faces of vL/vR x/y/z"}
 
 
 def _events(h, n):
@@ -446,6 +553,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--m", type=int, default=199)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the FP32/BF16-state leg (the metric's 'FP32-form speedup @ err')")
     ap.add_argument("--flags", type=int, default=0, help="extra weno_reconstruct flags")
     ap.add_argument("--suite", default=None, help="also run the secondary measurements into this JSON file")
     ap.add_argument("--adjoint", default=None,
@@ -564,7 +673,7 @@ def main():
     launch_ms = float(np.mean([x for p in per_dir for x in p]))
     achieved = bytes_launch / (launch_ms * 1e-3) / 1e9
     out = {
-        "metric": "WENO5 cell-faces/s, 200^3x8 FP64 (% HBM roofline)",
+        "metric": "WENO5 cell-faces/s, 200^3x8 FP64 (% HBM roofline); FP32-form speedup @ err",
         "value": value,
         "unit": "face values/s",
         "n_gpus": 1,
@@ -591,6 +700,12 @@ def main():
         L.free()
         R.free()
     vf.free()
+    if not args.no_precision:
+        prec = precision_leg(h, stream, m, v, buff, args.steps, args.warmup)
+        out["fp32_speedup"] = prec["fp32_speedup"]
+        out["fp32_max_rel_err"] = prec["fp32_max_rel_err"]
+        out["fp32_rms_rel_err"] = prec["fp32_rms_rel_err"]
+        out["fp32_form"] = prec
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(m, v, buff)
     if args.suite:

@@ -137,8 +137,12 @@ int weno_order1_copy(const weno_grid* g, int dir, int dtype, const void* v_vf, v
  * rest of vL/vR gets the order-1 values; with WENO_FUSE_MP the result is
  * additionally limited by s_preserve_monotonicity.
  * dtype: WENO_F64 (double field, double faces), WENO_F32 (float field,
- * float faces), WENO_BF16 (bf16 field, float faces); FP32/BF16 do the
- * stencil arithmetic in FP32 and beta and the weight normalisation in FP64. */
+ * float faces), WENO_BF16 (bf16 field, float faces).  WENO5-JS launches
+ * that write both sides over the core range (rows of at least 96 faces)
+ * widen FP32/BF16 state to FP64 and do all of the stencil arithmetic in
+ * FP64, rounding each face once; every other FP32/BF16 launch does the
+ * candidate polynomials in FP32 and beta and the weight normalisation in
+ * FP64. */
 int weno_reconstruct(const weno_grid* g, int order, int dir, int src_kind, int dtype,
                      const void* src, const weno_coeffs* c, void* vL, void* vR, int flags);
 

@@ -22,13 +22,22 @@ def rel_err(a, b, floor_frac=1e-3):
     b = np.asarray(b, dtype=np.float64)
     if a.shape != b.shape:
         raise AssertionError(f"shape {a.shape} != {b.shape}")
-    if not np.array_equal(np.isnan(a), np.isnan(b)):
-        return np.inf
-    m = ~np.isnan(b)
-    if not m.any():
+    nan_a, nan_b = np.isnan(a), np.isnan(b)
+    if nan_a.any() or nan_b.any():
+        if not np.array_equal(nan_a, nan_b):
+            return np.inf
+        m = ~nan_b
+        if not m.any():
+            return 0.0
+        a, b = a[m], b[m]
+    if b.size == 0:
         return 0.0
-    scale = np.maximum(np.abs(b[m]), floor_frac * np.max(np.abs(b[m])) + 1e-300)
-    return float(np.max(np.abs(a[m] - b[m]) / scale))
+    ab = np.abs(b)
+    scale = np.maximum(ab, floor_frac * ab.max() + 1e-300, out=ab)
+    d = np.subtract(a, b)
+    np.abs(d, out=d)
+    np.divide(d, scale, out=d)
+    return float(d.max())
 
 
 def stencil_scale(v_rs, R):

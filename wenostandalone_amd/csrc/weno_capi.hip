@@ -196,23 +196,6 @@ void plan_tiles(TileArgs& a, Launch& l, int np_override, int chunk_override, int
     const int halo = 8;                      // 2*R_max + 1, rounded up
     const long long lds_budget = 96 * 1024;  // both tile buffers
     const int max_np = kernel_max_np(), max_slots = kernel_max_slots();
-    int chunk = chunk_override > 0 ? std::min(chunk_override, a.out_len) : a.out_len;
-    const int min_np = std::min(np_target, 8);
-    if (2LL * min_np * (chunk + halo) * elem > lds_budget)
-        chunk = static_cast<int>(lds_budget / (2LL * min_np * elem)) - halo;
-    const int chunks = (a.out_len + chunk - 1) / chunk;
-    chunk = (a.out_len + chunks - 1) / chunks;
-    // lanes per group: the most outputs any chunk row has
-    int jn = 1;
-    for (int c = 0; c < chunks; ++c) {
-        const int jo0 = c * chunk, jo1 = std::min(a.out_len, jo0 + chunk);
-        const int c0 = std::max(jo0, a.j_beg), c1 = std::min(jo1 - 1, a.j_end);
-        const int n = a.copy_halo ? jo1 - jo0 : std::max(0, c1 - c0 + 1);
-        jn = std::max(jn, n);
-    }
-    // lane groups per block: best lane utilisation within 1024 threads
-    int G = 1, T = 64 * ((jn + 63) / 64);
-    double best = double(jn) / T;
     // FP64 mapped WENO5 (heavier arithmetic per face): 256-thread tiles, two
     // or more workgroups per CU (measured 1028 -> 984 us per 200^3 x 8 sweep)
     if (threads_cap <= 0 && l.op == OP_WENO && l.order == 5 && l.mode == MODE_MAPPED && l.dtype == DT_F64)
@@ -233,8 +216,29 @@ void plan_tiles(TileArgs& a, Launch& l, int np_override, int chunk_override, int
     // four (832) the lane-utilisation search would pick (measured 624 -> 609
     // us per 200^3 x 8 sweep)
     if (threads_cap <= 0 && l.op == OP_MP && l.dtype == DT_F64) threads_cap = kDefaultMpThreads;
-    const int tmax = threads_cap > 0 ? std::min(threads_cap, max_threads(l.op, l.order))
-                                     : max_threads(l.op, l.order);
+    // a whole number of waves, at most the kernel's workgroup limit
+    const int tmax = std::max(64, (threads_cap > 0 ? std::min(threads_cap, max_threads(l.op, l.order))
+                                                   : max_threads(l.op, l.order)) & ~63);
+    int chunk = chunk_override > 0 ? std::min(chunk_override, a.out_len) : a.out_len;
+    const int min_np = std::min(np_target, 8);
+    if (2LL * min_np * (chunk + halo) * elem > lds_budget)
+        chunk = static_cast<int>(lds_budget / (2LL * min_np * elem)) - halo;
+    // one lane per output of a chunk row: rows longer than the workgroup
+    // limit are split into more chunks
+    chunk = std::min(chunk, tmax);
+    const int chunks = (a.out_len + chunk - 1) / chunk;
+    chunk = (a.out_len + chunks - 1) / chunks;
+    // lanes per group: the most outputs any chunk row has
+    int jn = 1;
+    for (int c = 0; c < chunks; ++c) {
+        const int jo0 = c * chunk, jo1 = std::min(a.out_len, jo0 + chunk);
+        const int c0 = std::max(jo0, a.j_beg), c1 = std::min(jo1 - 1, a.j_end);
+        const int n = a.copy_halo ? jo1 - jo0 : std::max(0, c1 - c0 + 1);
+        jn = std::max(jn, n);
+    }
+    // lane groups per block: best lane utilisation within 1024 threads
+    int G = 1, T = 64 * ((jn + 63) / 64);
+    double best = double(jn) / T;
     auto search = [&](int cap) {
         G = 1;
         T = 64 * ((jn + 63) / 64);
