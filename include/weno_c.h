@@ -179,6 +179,20 @@ typedef struct weno_block {
  * an atomic counter, largest blocks first. */
 int weno_batched_reconstruct(const weno_block* blocks, int nblocks, int dir, int flags);
 
+/* weno_batched_reconstruct that allocates nothing.  No counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The work-item counter and the block descriptors live in a library-owned
+ * static device array (a 256-byte counter slot and WENO_NOALLOC_MAX_BLOCKS
+ * descriptors of 288 bytes: 73,984 bytes per GPU, held until the process
+ * ends) instead of memory the library allocates; more than
+ * WENO_NOALLOC_MAX_BLOCKS blocks is WENO_E_INVALID.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (other host threads, other streams) race.  Same kernels, same results as
+ * weno_batched_reconstruct. */
+#define WENO_NOALLOC_MAX_BLOCKS 256
+int weno_batched_reconstruct_noalloc(const weno_block* blocks, int nblocks, int dir, int flags);
+
 /* Captures the x, y and z batched launches into one hipGraph (the blocks'
  * arrays must stay allocated while the graph lives). */
 int weno_batched_graph_create(const weno_block* blocks, int nblocks, int flags, void** graph);

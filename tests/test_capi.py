@@ -90,3 +90,11 @@ def test_set_tile_validation():
     assert wm.lib.weno_set_tile(5, 0, 0, 16, 0, 100, -1) == -1     # not a multiple of 64
     assert wm.lib.weno_set_tile(5, 0, 0, 16, 0, 448, 0) == 0
     assert wm.lib.weno_set_tile(5, 0, 0, 0, 0, 0, -1) == 0
+
+
+def test_noalloc_batched_limit():
+    """The allocation-free batched launch refuses more blocks than its static
+    scratch holds, before touching the blocks or the GPU."""
+    blocks = (wm.Block * 257)()
+    rc = wm.lib.weno_batched_reconstruct_noalloc(blocks, 257, 0, wm.FUSE_COPY)
+    assert rc == -1 and "at most 256" in wm._lib.last_error()

@@ -99,6 +99,7 @@ PROTOTYPES = {
     "weno_reconstruct_vjp": (_i, [_G, _i, _i, _i, _vp, _CF, _vp, _vp, _vp]),
     "weno_stream_copy": (_i, [_vp, _vp, _sz]),
     "weno_batched_reconstruct": (_i, [_vp, _i, _i, _i]),
+    "weno_batched_reconstruct_noalloc": (_i, [_vp, _i, _i, _i]),
     "weno_batched_graph_create": (_i, [_vp, _i, _i, C.POINTER(_vp)]),
     "weno_sweep_graph_create": (_i, [_G, _i, _i, _i, _vp, _CF, C.POINTER(_vp), C.POINTER(_vp), _i,
                                      C.POINTER(_vp)]),

@@ -24,6 +24,18 @@ int batch_max_threads();
 int batch_max_np();
 void set_tuned(int order, int dtype, int dir, int np, int chunk, int threads, int prefetch);
 
+// Allocation-free batched launches keep the work-item counter and the block
+// descriptors in this file-scope device array instead of library-allocated
+// memory: the counter in the first 256 bytes, then up to kNoallocMaxBlocks
+// descriptors.  Each GPU holds its own copy until the process ends.  In host
+// code `scratch` names a host-side stand-in, never the device memory: the
+// device address comes from hipGetSymbolAddress, once per device.
+constexpr int kNoallocMaxBlocks = WENO_NOALLOC_MAX_BLOCKS;
+constexpr size_t kNoallocCounterBytes = 256;
+constexpr size_t kNoallocScratchBytes = kNoallocCounterBytes + size_t(kNoallocMaxBlocks) * sizeof(BatchDesc);
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+static_assert(sizeof(BatchDesc) == 288, "the header states the scratch size from a 288-byte descriptor");
+
 namespace {
 
 struct BatchTune {
@@ -51,73 +63,91 @@ struct GraphHandle {
 // coefficient table row stride (BetaTraits::LD) per order
 int entries_of(int order) { return order == 3 ? 7 : (order == 5 ? 15 : 33); }
 
+// One block's launch description along dir; cost orders the work queue
+// (largest blocks first) and lds is the dynamic LDS the block's tiles need.
+int describe_block(const weno_block& B, int b, int dir, int flags, int threads, int np_target, BatchDesc& D,
+                   double& cost, size_t& lds)
+{
+    int rc;
+    const weno_grid* g = &B.grid;
+    if ((rc = check_grid(g))) return fail(rc, "block %d: %s", b, weno_last_error());
+    if (B.order != 3 && B.order != 5 && B.order != 7)
+        return fail(WENO_E_INVALID, "block %d: order must be 3, 5 or 7 (got %d)", b, B.order);
+    const int mp = (flags & WENO_FUSE_MP) ? 1 : 0;
+    const Dims d = dims_of(g, dir);
+    std::memset(&D, 0, sizeof D);
+    TileArgs& a = D.a;
+    const void* s;
+    void *oL, *oR;
+    if ((rc = resolve(B.v_vf, &s, "v_vf")) || (rc = resolve_mut(B.vL[dir], &oL, "vL")) ||
+        (rc = resolve_mut(B.vR[dir], &oR, "vR")) || (rc = resolve_coeffs(&B.coeffs[dir], a.c, d.ns - 6)))
+        return fail(rc, "block %d: %s", b, weno_last_error());
+    input_map(d, dir, WENO_SRC_FIELD, a);
+    output_map(d, d.ns, a);
+    a.src = s;
+    a.outL = oL;
+    a.outR = oR;
+    a.J0f = 3;
+    a.j_beg = B.order == 3 ? 0 : 3;
+    a.j_end = B.order == 3 ? d.ns - 1 : d.ns - 4;
+    a.mp_beg = 3;
+    a.mp_end = d.ns - 4;
+    a.copy_halo = (flags & WENO_FUSE_COPY) ? 1 : 0;
+    a.sides = 3;
+    a.eps = g->weno_eps;
+    a.teno_ct = g->teno_CT;
+    a.chunk = d.ns;
+    a.chunks = 1;
+    const int R = std::max(B.order / 2, mp ? 2 : 0);
+    const int jn = a.copy_halo ? d.ns : a.j_end - a.j_beg + 1;
+    if (jn > threads)
+        return fail(WENO_E_INVALID, "block %d: sweep length %d exceeds the batched workgroup (%d threads)", b, jn,
+                    threads);
+    const int lo = a.copy_halo ? std::min(3, a.j_beg + 3 - R) : a.j_beg + 3 - R;
+    const int hi = a.copy_halo ? std::max(d.ns + 2, a.j_end + 3 + R) : a.j_end + 3 + R;
+    const int ldj = (hi - lo + 1) | 1;
+    const int G = std::max(1, threads / jn);
+    int npg = std::max(1, np_target / G);
+    npg = std::min(npg, std::max(1, batch_max_np() / G));
+    auto lds_of = [&](int np) { return (size_t(np) * ldj + size_t(entries_of(B.order)) * jn) * sizeof(double); };
+    while (npg > 1 && lds_of(G * npg) > 96 * 1024) --npg;
+    a.groups = std::min(G, batch_max_np());
+    a.npg = npg;
+    a.np = a.groups * npg;
+    a.jn = jn;
+    a.ldj = ldj;
+    a.threads = threads;
+    a.tiles_per_var = static_cast<int>((static_cast<long long>(a.na) * a.nb + a.np - 1) / a.np);
+    D.order = B.order;
+    D.mode = mode_for(g, B.order);
+    lds = lds_of(a.np);
+    cost = double(a.tiles_per_var) * a.nv * a.np * jn;
+    return WENO_OK;
+}
+
+int check_batch_args(const weno_block* blocks, int nblocks, int flags)
+{
+    if (!blocks || nblocks <= 0) return fail(WENO_E_INVALID, "no blocks");
+    if (flags & ~(WENO_FUSE_MP | WENO_FUSE_COPY))
+        return fail(WENO_E_INVALID, "batched flags: only WENO_FUSE_MP and WENO_FUSE_COPY (got 0x%x)", flags);
+    return WENO_OK;
+}
+
 int plan_batch(const weno_block* blocks, int nblocks, int dir, int flags, int threads, int np_target,
                BatchPlan& plan)
 {
     int rc;
-    if (!blocks || nblocks <= 0) return fail(WENO_E_INVALID, "no blocks");
-    if (flags & ~(WENO_FUSE_MP | WENO_FUSE_COPY))
-        return fail(WENO_E_INVALID, "batched flags: only WENO_FUSE_MP and WENO_FUSE_COPY (got 0x%x)", flags);
+    if ((rc = check_batch_args(blocks, nblocks, flags))) return rc;
     plan.descs.clear();
     plan.threads = threads;
     plan.mp = (flags & WENO_FUSE_MP) ? 1 : 0;
-    std::vector<std::pair<double, BatchDesc>> work;
+    plan.lds = 0;
+    std::vector<std::pair<double, BatchDesc>> work(nblocks);
     for (int b = 0; b < nblocks; ++b) {
-        const weno_block& B = blocks[b];
-        const weno_grid* g = &B.grid;
-        if ((rc = check_grid(g))) return fail(rc, "block %d: %s", b, weno_last_error());
-        if (B.order != 3 && B.order != 5 && B.order != 7)
-            return fail(WENO_E_INVALID, "block %d: order must be 3, 5 or 7 (got %d)", b, B.order);
-        const Dims d = dims_of(g, dir);
-        BatchDesc D;
-        std::memset(&D, 0, sizeof D);
-        TileArgs& a = D.a;
-        const void* s;
-        void *oL, *oR;
-        if ((rc = resolve(B.v_vf, &s, "v_vf")) || (rc = resolve_mut(B.vL[dir], &oL, "vL")) ||
-            (rc = resolve_mut(B.vR[dir], &oR, "vR")) || (rc = resolve_coeffs(&B.coeffs[dir], a.c, d.ns - 6)))
-            return fail(rc, "block %d: %s", b, weno_last_error());
-        input_map(d, dir, WENO_SRC_FIELD, a);
-        output_map(d, d.ns, a);
-        a.src = s;
-        a.outL = oL;
-        a.outR = oR;
-        a.J0f = 3;
-        a.j_beg = B.order == 3 ? 0 : 3;
-        a.j_end = B.order == 3 ? d.ns - 1 : d.ns - 4;
-        a.mp_beg = 3;
-        a.mp_end = d.ns - 4;
-        a.copy_halo = (flags & WENO_FUSE_COPY) ? 1 : 0;
-        a.sides = 3;
-        a.eps = g->weno_eps;
-        a.teno_ct = g->teno_CT;
-        a.chunk = d.ns;
-        a.chunks = 1;
-        const int R = std::max(B.order / 2, plan.mp ? 2 : 0);
-        const int jn = a.copy_halo ? d.ns : a.j_end - a.j_beg + 1;
-        if (jn > threads)
-            return fail(WENO_E_INVALID, "block %d: sweep length %d exceeds the batched workgroup (%d threads)", b, jn,
-                        threads);
-        const int lo = a.copy_halo ? std::min(3, a.j_beg + 3 - R) : a.j_beg + 3 - R;
-        const int hi = a.copy_halo ? std::max(d.ns + 2, a.j_end + 3 + R) : a.j_end + 3 + R;
-        const int ldj = (hi - lo + 1) | 1;
-        const int G = std::max(1, threads / jn);
-        int npg = std::max(1, np_target / G);
-        npg = std::min(npg, std::max(1, batch_max_np() / G));
-        auto lds_of = [&](int np) { return (size_t(np) * ldj + size_t(entries_of(B.order)) * jn) * sizeof(double); };
-        while (npg > 1 && lds_of(G * npg) > 96 * 1024) --npg;
-        a.groups = std::min(G, batch_max_np());
-        a.npg = npg;
-        a.np = a.groups * npg;
-        a.jn = jn;
-        a.ldj = ldj;
-        a.threads = threads;
-        a.tiles_per_var = static_cast<int>((static_cast<long long>(a.na) * a.nb + a.np - 1) / a.np);
-        D.order = B.order;
-        D.mode = mode_for(g, B.order);
-        plan.lds = std::max(plan.lds, lds_of(a.np));
-        const double cost = double(a.tiles_per_var) * a.nv * a.np * jn;
-        work.emplace_back(cost, D);
+        size_t lds = 0;
+        if ((rc = describe_block(blocks[b], b, dir, flags, threads, np_target, work[b].second, work[b].first, lds)))
+            return rc;
+        plan.lds = std::max(plan.lds, lds);
     }
     // largest blocks first: their items are pulled first and finish last
     std::stable_sort(work.begin(), work.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
@@ -410,6 +440,71 @@ int weno_batched_autotune(const weno_block* blocks, int nblocks, int flags, int*
         best[2] = static_cast<int>(best_ms * 1000.f);
     }
     return WENO_OK;
+}
+
+int weno_batched_reconstruct_noalloc(const weno_block* blocks, int nblocks, int dir, int flags)
+{
+    int rc;
+    if ((rc = check_dir(dir))) return rc;
+    if ((rc = check_batch_args(blocks, nblocks, flags))) return rc;
+    if (nblocks > kNoallocMaxBlocks)
+        return fail(WENO_E_INVALID, "allocation-free batched launch: %d blocks, at most %d", nblocks,
+                    kNoallocMaxBlocks);
+    if ((rc = ensure_init())) return rc;
+    // one call at a time per process: the host staging below and each GPU's
+    // scratch array are shared by every call
+    static std::mutex mu;
+    static void* dev_scratch[64] = {};
+    static BatchDesc described[kNoallocMaxBlocks], queued[kNoallocMaxBlocks];
+    static double cost[kNoallocMaxBlocks];
+    static int order_of[kNoallocMaxBlocks];
+    std::lock_guard<std::mutex> lk(mu);
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (dev < 0 || dev >= 64) return fail(WENO_E_INVALID, "device %d out of range for the scratch table", dev);
+    if (!dev_scratch[dev]) {
+        void* p = nullptr;
+        if ((e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch))) != hipSuccess)
+            return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        dev_scratch[dev] = p;
+    }
+    BatchTune t;
+    {
+        std::lock_guard<std::mutex> tl(g_bmu);
+        t = g_btune;
+    }
+    size_t lds = 0;
+    for (int b = 0; b < nblocks; ++b) {
+        size_t need = 0;
+        if ((rc = describe_block(blocks[b], b, dir, flags, t.threads, t.np_target, described[b], cost[b], need)))
+            return rc;
+        lds = std::max(lds, need);
+    }
+    // largest blocks first (stable for equal costs), as weno_batched_reconstruct
+    for (int b = 0; b < nblocks; ++b) {
+        int j = b;
+        while (j > 0 && cost[order_of[j - 1]] < cost[b]) {
+            order_of[j] = order_of[j - 1];
+            --j;
+        }
+        order_of[j] = b;
+    }
+    int total = 0;
+    for (int k = 0; k < nblocks; ++k) {
+        queued[k] = described[order_of[k]];
+        queued[k].item0 = total;
+        total += queued[k].a.tiles_per_var * queued[k].a.nv;
+    }
+    char* const base = static_cast<char*>(dev_scratch[dev]);
+    BatchDesc* const descs = reinterpret_cast<BatchDesc*>(base + kNoallocCounterBytes);
+    int* const counter = reinterpret_cast<int*>(base);
+    if ((e = hipMemcpyAsync(descs, queued, size_t(nblocks) * sizeof(BatchDesc), hipMemcpyHostToDevice, stream())) !=
+        hipSuccess)
+        return hip_fail(e, "descriptor upload");
+    return finish(launch_batched(descs, nblocks, total, counter, lds, t.threads, (flags & WENO_FUSE_MP) ? 1 : 0,
+                                 stream()),
+                  "batched (allocation-free)");
 }
 
 }  // extern "C"
