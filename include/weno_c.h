@@ -157,8 +157,9 @@ int weno_mp_limit(const weno_grid* g, int dir, int src_kind, int dtype, const vo
  * to gv, which has src's layout (v_vf or v_rs_ws_dir) and must be zeroed by
  * the caller before the first sweep; beta and the weights are recomputed,
  * nothing is stored by the forward pass.  Sweeps of one field accumulate.
- * Sweeps of up to 512 core points (ns <= 518) are supported; longer ones
- * return WENO_E_INVALID. */
+ * Sweeps of up to 512 core points (ns <= 518) for WENO5 and 479 core
+ * points (ns <= 485) for WENO7 are supported; longer ones return
+ * WENO_E_INVALID. */
 int weno_reconstruct_vjp(const weno_grid* g, int order, int dir, int src_kind, const void* src,
                          const weno_coeffs* c, const void* gL, const void* gR, void* gv);
 

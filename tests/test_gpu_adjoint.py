@@ -146,7 +146,8 @@ def test_vjp_central_differences_16cube(order):
 
 @pytest.mark.parametrize("order", [5, 7])
 @pytest.mark.parametrize("m,n,p,v,dirs", [
-    (47, 47, 47, 8, (0, 1, 2)),   # 256-thread workgroups, two per CU, ~4 tiles each
+    (47, 47, 47, 8, (0, 1, 2)),   # 54-point rows: 64-thread workgroups of one lane group
+    (243, 17, 17, 2, (0,)),       # 250-point rows: 256-thread workgroups, two per CU
     (127, 59, 59, 4, (0,)),       # 135-point rows: 448-thread workgroups, ~6 tiles each
 ])
 def test_vjp_many_tiles_per_workgroup(order, m, n, p, v, dirs):
@@ -171,3 +172,31 @@ def test_vjp_many_tiles_per_workgroup(order, m, n, p, v, dirs):
         wm.check(wm.lib.weno_reconstruct_vjp(C.byref(g), order, d, wm.SRC_RESHAPED, rs.vp, C.byref(coef),
                                              dgL.vp, dgR.vp, gv.vp))
         assert norm_err(gv.download() - 0.25, ref_rs) <= 1e-10, d
+
+
+@pytest.mark.parametrize("order,m", [(5, 505), (7, 472)])
+def test_vjp_longest_supported_sweep(order, m):
+    """The longest sweeps the adjoint supports (WENO5 512, WENO7 479 core
+    points) against the oracle; one more point is refused."""
+    b, n, p, v, d = 6, 3, 3, 1, 0
+    g = wm.make_grid(m, n, p, v=v, buff=b)
+    og = oracle.grid(m, n, p, v=v, buff=b)
+    f = fields.seeded_field(m, n, p, v, b, seed=95 + order)
+    ns, nk, nl = fields.dir_extents(m, n, p, b, d)
+    cs = fields.harness_coefficients(ns - 6, fields.DIR_OFFSETS[d])
+    devs = [DeviceArray.from_host(c) for c in cs]
+    coef = wm.Coeffs(*[x.ptr for x in devs])
+    gL, gR = cotangents((ns, nk, nl, v), seed=7)
+    v_rs = oracle.reshape(og, d, f)
+    ref_rs = oracle.vjp(og, order, d, v_rs, cs, gL, gR)
+    dgL, dgR = DeviceArray.from_host(gL), DeviceArray.from_host(gR)
+    rs = DeviceArray.from_host(v_rs)
+    gv = DeviceArray(v_rs.shape)
+    gv.fill(0.0)
+    wm.check(wm.lib.weno_reconstruct_vjp(C.byref(g), order, d, wm.SRC_RESHAPED, rs.vp, C.byref(coef),
+                                         dgL.vp, dgR.vp, gv.vp))
+    assert norm_err(gv.download(), ref_rs) <= 1e-10
+    g1 = wm.make_grid(m + 1, n, p, v=v, buff=b)
+    rc = wm.lib.weno_reconstruct_vjp(C.byref(g1), order, d, wm.SRC_RESHAPED, rs.vp, C.byref(coef),
+                                     dgL.vp, dgR.vp, gv.vp)
+    assert rc == -1 and "core points" in wm._lib.last_error()

@@ -25,10 +25,20 @@ extern "C" int weno_reconstruct_vjp(const weno_grid* g, int order, int dir, int 
     const Dims d = dims_of(g, dir);
     TileArgs a;
     std::memset(&a, 0, sizeof a);
+    {
+        // the longest sweeps whose one-pencil tile fits LDS: WENO5 the
+        // workgroup limit (512), WENO7 479 core points (its 33-entry
+        // coefficient table per lane takes most of the 158 KB)
+        const int max_core = order == 5 ? vjp_max_threads() : 479;
+        if (d.ns - 6 > max_core)
+            return fail(WENO_E_INVALID, "WENO%d adjoint sweep length %d exceeds %d core points", order, d.ns - 6,
+                        max_core);
+    }
     const void *s, *l, *r;
     void* o;
-    if ((rc = resolve(src, &s, "src")) || (rc = resolve(gL, &l, "gL")) || (rc = resolve(gR, &r, "gR")) ||
-        (rc = resolve_mut(gv, &o, "gv")))
+    const size_t sb = src_elems(d, src_kind) * sizeof(double), fb = face_elems(d) * sizeof(double);
+    if ((rc = resolve(src, &s, "src", sb)) || (rc = resolve(gL, &l, "gL", fb)) || (rc = resolve(gR, &r, "gR", fb)) ||
+        (rc = resolve_mut(gv, &o, "gv", sb)))
         return rc;
     if ((rc = resolve_coeffs(c, a.c, d.ns - 6))) return rc;
     input_map(d, dir, src_kind, a);
@@ -46,7 +56,6 @@ extern "C" int weno_reconstruct_vjp(const weno_grid* g, int order, int dir, int 
     const int jn = a.j_end - a.j_beg + 1;
     const int nJ = jn + 2 * R;
     const int tmax = vjp_max_threads();
-    if (jn > tmax) return fail(WENO_E_INVALID, "adjoint sweep length %d exceeds %d core points", jn, tmax);
     // G lane groups of jn lanes.  The kernel's register use allows 8 waves
     // per CU: workgroups of at most 4 waves run two per CU, so that one
     // loads while the other computes; take that shape when it keeps at

@@ -54,8 +54,10 @@ int finish(hipError_t e, const char* what)
 
 // Translate a user pointer into a device address: a host address present in
 // the data environment maps to its device copy; otherwise the pointer must be
-// device memory.
-int resolve(const void* p, const void** out, const char* name)
+// device memory.  With need > 0 the `need` bytes from p must lie inside the
+// mapping (OpenMP's present check rejects a partly present array) or inside
+// the device allocation, so that no launch reaches past either.
+int resolve(const void* p, const void** out, const char* name, size_t need)
 {
     if (p == nullptr) return fail(WENO_E_INVALID, "%s is NULL", name);
     {
@@ -65,6 +67,10 @@ int resolve(const void* p, const void** out, const char* name)
             --it;
             const uintptr_t off = reinterpret_cast<uintptr_t>(p) - it->first;
             if (off < it->second.bytes) {
+                if (need > it->second.bytes - off)
+                    return fail(WENO_E_INVALID,
+                                "%s (%p) is present with %zu bytes from that address; the call needs %zu", name, p,
+                                it->second.bytes - off, need);
                 *out = static_cast<const char*>(it->second.dev) + off;
                 return WENO_OK;
             }
@@ -73,6 +79,20 @@ int resolve(const void* p, const void** out, const char* name)
     hipPointerAttribute_t attr;
     hipError_t e = hipPointerGetAttributes(&attr, p);
     if (e == hipSuccess && (attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged)) {
+        if (need > 0) {
+            hipDeviceptr_t base = nullptr;
+            size_t size = 0;
+            if (hipMemGetAddressRange(&base, &size, const_cast<void*>(p)) != hipSuccess || !base) {
+                (void)hipGetLastError();
+                return fail(WENO_E_INVALID, "%s (%p): the extent of this device allocation cannot be determined",
+                            name, p);
+            }
+            const size_t off = static_cast<size_t>(static_cast<const char*>(p) - static_cast<const char*>(base));
+            if (off > size || need > size - off)
+                return fail(WENO_E_INVALID,
+                            "%s (%p) has %zu bytes of device memory from that address; the call needs %zu", name, p,
+                            off > size ? size_t(0) : size - off, need);
+        }
         *out = p;
         return WENO_OK;
     }
@@ -81,13 +101,23 @@ int resolve(const void* p, const void** out, const char* name)
                 "%s (%p) is neither device memory nor present in the data environment", name, p);
 }
 
-int resolve_mut(void* p, void** out, const char* name)
+int resolve_mut(void* p, void** out, const char* name, size_t need)
 {
     const void* q = nullptr;
-    int rc = resolve(p, &q, name);
+    int rc = resolve(p, &q, name, need);
     *out = const_cast<void*>(q);
     return rc;
 }
+
+size_t field_elems(const Dims& d) { return size_t(d.nx + 6) * size_t(d.ny + 6) * size_t(d.nz + 6) * size_t(d.nv); }
+size_t reshaped_elems(const Dims& d) { return size_t(d.ns + 6) * size_t(d.nk) * size_t(d.nl) * size_t(d.nv); }
+size_t face_elems(const Dims& d) { return size_t(d.ns) * size_t(d.nk) * size_t(d.nl) * size_t(d.nv); }
+size_t src_elems(const Dims& d, int src_kind)
+{
+    return src_kind == WENO_SRC_RESHAPED ? reshaped_elems(d) : field_elems(d);
+}
+int state_bytes(int dtype) { return dtype == WENO_F64 ? 8 : (dtype == WENO_F32 ? 4 : 2); }
+int face_bytes(int dtype) { return dtype == WENO_F64 ? 8 : 4; }
 
 int check_grid(const weno_grid* g)
 {
@@ -311,15 +341,17 @@ int resolve_coeffs(const weno_coeffs* c, Coefs& out, int nc)
     if (!c) return fail(WENO_E_INVALID, "coefficients are NULL");
     const void* p;
     int rc;
-    if ((rc = resolve(c->polyL, &p, "polyL"))) return rc;
+    // poly (nc,5,3), d (5,nc), beta (nc,5,6) doubles (:171-187)
+    const size_t n = static_cast<size_t>(std::max(nc, 0)) * sizeof(double);
+    if ((rc = resolve(c->polyL, &p, "polyL", 15 * n))) return rc;
     out.polyL = static_cast<const double*>(p);
-    if ((rc = resolve(c->polyR, &p, "polyR"))) return rc;
+    if ((rc = resolve(c->polyR, &p, "polyR", 15 * n))) return rc;
     out.polyR = static_cast<const double*>(p);
-    if ((rc = resolve(c->dL, &p, "dL"))) return rc;
+    if ((rc = resolve(c->dL, &p, "dL", 5 * n))) return rc;
     out.dL = static_cast<const double*>(p);
-    if ((rc = resolve(c->dR, &p, "dR"))) return rc;
+    if ((rc = resolve(c->dR, &p, "dR", 5 * n))) return rc;
     out.dR = static_cast<const double*>(p);
-    if ((rc = resolve(c->beta, &p, "beta_coef"))) return rc;
+    if ((rc = resolve(c->beta, &p, "beta_coef", 30 * n))) return rc;
     out.beta = static_cast<const double*>(p);
     out.nc = nc;
     return WENO_OK;
@@ -350,8 +382,9 @@ int build_reconstruct(const weno_grid* g, int order, int dir, int src_kind, int 
     const Dims d = dims_of(g, dir);
     const void* s;
     void *oL, *oR;
-    if ((rc = resolve(src, &s, "src")) || (rc = resolve_mut(vL, &oL, "vL")) ||
-        (rc = resolve_mut(vR, &oR, "vR")))
+    const size_t fb = face_elems(d) * face_bytes(dtype);
+    if ((rc = resolve(src, &s, "src", src_elems(d, src_kind) * state_bytes(dtype))) ||
+        (rc = resolve_mut(vL, &oL, "vL", fb)) || (rc = resolve_mut(vR, &oR, "vR", fb)))
         return rc;
     if ((rc = resolve_coeffs(c, a.c, d.ns - 6))) return rc;
     input_map(d, dir, src_kind, a);
@@ -640,7 +673,9 @@ int weno_reshape(const weno_grid* g, int dir, int dtype, const void* v_vf, void*
     const Dims d = dims_of(g, dir);
     const void* s;
     void* o;
-    if ((rc = resolve(v_vf, &s, "v_vf")) || (rc = resolve_mut(v_rs, &o, "v_rs_ws"))) return rc;
+    if ((rc = resolve(v_vf, &s, "v_vf", field_elems(d) * state_bytes(dtype))) ||
+        (rc = resolve_mut(v_rs, &o, "v_rs_ws", reshaped_elems(d) * state_bytes(dtype))))
+        return rc;
     input_map(d, dir, WENO_SRC_FIELD, a);
     output_map(d, d.ns + 6, a);
     a.src = s;
@@ -668,8 +703,9 @@ int weno_order1_copy(const weno_grid* g, int dir, int dtype, const void* v_vf, v
     const Dims d = dims_of(g, dir);
     const void* s;
     void *oL, *oR;
-    if ((rc = resolve(v_vf, &s, "v_vf")) || (rc = resolve_mut(vL, &oL, "vL")) ||
-        (rc = resolve_mut(vR, &oR, "vR")))
+    const size_t fb = face_elems(d) * face_bytes(dtype);
+    if ((rc = resolve(v_vf, &s, "v_vf", field_elems(d) * state_bytes(dtype))) ||
+        (rc = resolve_mut(vL, &oL, "vL", fb)) || (rc = resolve_mut(vR, &oR, "vR", fb)))
         return rc;
     input_map(d, dir, WENO_SRC_FIELD, a);
     output_map(d, d.ns, a);
@@ -711,8 +747,9 @@ int weno_mp_limit(const weno_grid* g, int dir, int src_kind, int dtype, const vo
     const Dims d = dims_of(g, dir);
     const void* s;
     void *oL, *oR;
-    if ((rc = resolve(src, &s, "src")) || (rc = resolve_mut(vL, &oL, "vL")) ||
-        (rc = resolve_mut(vR, &oR, "vR")))
+    const size_t fb = face_elems(d) * face_bytes(dtype);
+    if ((rc = resolve(src, &s, "src", src_elems(d, src_kind) * state_bytes(dtype))) ||
+        (rc = resolve_mut(vL, &oL, "vL", fb)) || (rc = resolve_mut(vR, &oR, "vR", fb)))
         return rc;
     input_map(d, dir, src_kind, a);
     output_map(d, d.ns, a);

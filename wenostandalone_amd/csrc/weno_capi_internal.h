@@ -33,8 +33,16 @@ int hip_fail(hipError_t e, const char* what);
 int ensure_init();
 hipStream_t stream();
 int finish(hipError_t e, const char* what);
-int resolve(const void* p, const void** out, const char* name);
-int resolve_mut(void* p, void** out, const char* name);
+// need: bytes from p that the launch will touch (0: not checked)
+int resolve(const void* p, const void** out, const char* name, size_t need = 0);
+int resolve_mut(void* p, void** out, const char* name, size_t need = 0);
+// element counts of the reference's arrays (weno_standalone.f90:163-187)
+size_t field_elems(const Dims& d);
+size_t reshaped_elems(const Dims& d);
+size_t face_elems(const Dims& d);
+size_t src_elems(const Dims& d, int src_kind);
+int state_bytes(int dtype);   // v_vf / v_rs_ws element size
+int face_bytes(int dtype);    // vL / vR element size
 int check_grid(const weno_grid* g);
 int check_dir(int dir);
 int check_dtype(int dtype);

@@ -79,8 +79,10 @@ int describe_block(const weno_block& B, int b, int dir, int flags, int threads, 
     TileArgs& a = D.a;
     const void* s;
     void *oL, *oR;
-    if ((rc = resolve(B.v_vf, &s, "v_vf")) || (rc = resolve_mut(B.vL[dir], &oL, "vL")) ||
-        (rc = resolve_mut(B.vR[dir], &oR, "vR")) || (rc = resolve_coeffs(&B.coeffs[dir], a.c, d.ns - 6)))
+    const size_t fb = face_elems(d) * sizeof(double);
+    if ((rc = resolve(B.v_vf, &s, "v_vf", field_elems(d) * sizeof(double))) ||
+        (rc = resolve_mut(B.vL[dir], &oL, "vL", fb)) || (rc = resolve_mut(B.vR[dir], &oR, "vR", fb)) ||
+        (rc = resolve_coeffs(&B.coeffs[dir], a.c, d.ns - 6)))
         return fail(rc, "block %d: %s", b, weno_last_error());
     input_map(d, dir, WENO_SRC_FIELD, a);
     output_map(d, d.ns, a);

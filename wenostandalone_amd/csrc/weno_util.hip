@@ -34,7 +34,7 @@ extern "C" int weno_stream_copy(void* dst, const void* src, size_t bytes)
     if (bytes % 16) return fail(WENO_E_INVALID, "stream copy size must be a multiple of 16 bytes");
     const void* s;
     void* d;
-    if ((rc = resolve(src, &s, "src")) || (rc = resolve_mut(dst, &d, "dst"))) return rc;
+    if ((rc = resolve(src, &s, "src", bytes)) || (rc = resolve_mut(dst, &d, "dst", bytes))) return rc;
     if ((reinterpret_cast<uintptr_t>(s) | reinterpret_cast<uintptr_t>(d)) & 15)
         return fail(WENO_E_INVALID, "stream copy needs 16-byte aligned buffers");
     const size_t n = bytes / 16;
