@@ -223,7 +223,7 @@ extern "C" {
 int weno_batched_reconstruct(const weno_block* blocks, int nblocks, int dir, int flags)
 {
     int rc;
-    if ((rc = check_dir(dir))) return rc;
+    if ((rc = check_dir(dir)) || (rc = check_batch_args(blocks, nblocks, flags))) return rc;
     if ((rc = ensure_init())) return rc;
     BatchTune t;
     {

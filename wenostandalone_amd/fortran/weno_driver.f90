@@ -5,7 +5,10 @@
 !!
 !! Extra switches: WENO_MAPPED=1, WENO_Z=1, WENO_TENO=1 (mapped-WENO, WENO-Z
 !! or TENO weights when wenojs=F, in that precedence), WENO5_FUSED=1
-!! (weno5 mode: one pass per sweep from v_vf, order-1 halo included).
+!! (weno5 mode: one pass per sweep from v_vf, order-1 halo included),
+!! WENO_METRICS_JSON=<path> (the run's configuration, checksums and times
+!! as one JSON object in that file; non-finite checksums are written as
+!! null, and "status" says which of the three closing lines was printed).
 program weno_test_driver
 
     use, intrinsic :: iso_c_binding
@@ -24,6 +27,7 @@ program weno_test_driver
     character(len=16) :: kernel_mode
     character(len=16) :: weno5_bench_scope
     character(len=256) :: input_file
+    character(len=256) :: metrics_path
     integer(c_size_t) :: nb
 
     input_file = 'inputs/weno_input.nml'
@@ -143,6 +147,9 @@ program weno_test_driver
         write(*,'(a)') '  All kernel families executed with finite non-zero output.'
     end if
 
+    call s_get_env_string('WENO_METRICS_JSON', '', metrics_path)
+    if (len_trim(metrics_path) > 0) call s_write_metrics(trim(metrics_path))
+
     call weno_check(weno_target_exit_data(c_loc(vL_x)), 'exit data')
     call weno_check(weno_target_exit_data(c_loc(vR_x)), 'exit data')
     call weno_check(weno_target_exit_data(c_loc(vL_y)), 'exit data')
@@ -152,5 +159,68 @@ program weno_test_driver
     deallocate(vL_x, vR_x, vL_y, vR_y, vL_z, vR_z)
     call s_finalize_data()
     call weno_check(weno_finalize(), 'finalize')
+
+contains
+
+    !> JSON value of a real: the number, or null when it is not finite.
+    function json_real(x) result(txt)
+        real(wp), intent(in) :: x
+        character(len=32) :: txt
+        if (ieee_is_finite(x)) then
+            write(txt, '(es24.16)') x
+            txt = adjustl(txt)
+        else
+            txt = 'null'
+        end if
+    end function json_real
+
+    function json_bool(b) result(txt)
+        logical, intent(in) :: b
+        character(len=5) :: txt
+        txt = merge('true ', 'false', b)
+    end function json_bool
+
+    subroutine s_write_metrics(path)
+        character(len=*), intent(in) :: path
+        integer :: iu, ios
+        character(len=16) :: status
+        if (.not. ieee_is_finite(chk_total)) then
+            status = 'non-finite'
+        else if (chk_total == 0.0_wp) then
+            status = 'all-zero'
+        else
+            status = 'ok'
+        end if
+        open(newunit=iu, file=path, status='replace', action='write', iostat=ios)
+        if (ios /= 0) then
+            write(*,'(3a)') '  WARNING: could not write metrics to "', path, '"'
+            return
+        end if
+        write(iu,'(a)') '{'
+        write(iu,'(3a)') '  "kernel_mode": "', trim(kernel_mode), '",'
+        write(iu,'(3a)') '  "weno5_bench_scope": "', trim(weno5_bench_scope), '",'
+        write(iu,'(3a)') '  "weno5_split_kernels": ', trim(json_bool(weno5_split_kernels)), ','
+        write(iu,'(3a)') '  "weno5_specialized_combined": ', trim(json_bool(weno5_specialized_combined)), ','
+        write(iu,'(3a)') '  "weno5_fused": ', trim(json_bool(weno5_fused)), ','
+        write(iu,'(a,i0,a)') '  "m_cells": ', m_cells, ','
+        write(iu,'(a,i0,a)') '  "n_cells": ', n_cells, ','
+        write(iu,'(a,i0,a)') '  "p_cells": ', p_cells, ','
+        write(iu,'(a,i0,a)') '  "v_size": ', v_size, ','
+        write(iu,'(a,i0,a)') '  "buff_size": ', buff_size, ','
+        write(iu,'(3a)') '  "weno_eps": ', trim(json_real(weno_eps)), ','
+        write(iu,'(3a)') '  "wenojs": ', trim(json_bool(wenojs)), ','
+        write(iu,'(3a)') '  "teno_CT": ', trim(json_real(teno_CT)), ','
+        write(iu,'(a,i0,a)') '  "warmup_iters": ', warmup_iters, ','
+        write(iu,'(a,i0,a)') '  "bench_iters": ', benchmark_iters, ','
+        write(iu,'(3a)') '  "checksum_x": ', trim(json_real(chk_x)), ','
+        write(iu,'(3a)') '  "checksum_y": ', trim(json_real(chk_y)), ','
+        write(iu,'(3a)') '  "checksum_z": ', trim(json_real(chk_z)), ','
+        write(iu,'(3a)') '  "checksum_t": ', trim(json_real(chk_total)), ','
+        write(iu,'(3a)') '  "kernel_time_total_s": ', trim(json_real(elapsed_total)), ','
+        write(iu,'(3a)') '  "kernel_time_per_iter_s": ', trim(json_real(elapsed_average)), ','
+        write(iu,'(3a)') '  "status": "', trim(status), '"'
+        write(iu,'(a)') '}'
+        close(iu)
+    end subroutine s_write_metrics
 
 end program weno_test_driver
