@@ -546,6 +546,62 @@ def families(path, m=199, v=8, buff=6, reps=10):
     return out
 
 
+def consumer(path, m=199, v=8, buff=6, reps=20):
+    """The consumer step (flux divergence of one sweep's faces) at m+1 cubed
+    x v FP64, WENO5-JS, Burgers flux, x/y/z: the fused pass
+    (weno_reconstruct_flux: state read once, rhs written once) against
+    reconstruction followed by the separate flux pass (weno_reconstruct +
+    weno_flux_divergence).  Algorithmic bytes per sweep with ns = m+1+2*buff
+    points along it and ns^2 pencils of v variables:
+      fused ......... read (ns-6+4) inputs, write (ns-8) rhs per pencil
+      two passes .... reconstruction: read (ns-2), write 2 (ns-6) faces;
+                      flux: read 2 (ns-6) faces, write (ns-8) rhs"""
+    import wenostandalone_amd as wm
+    from wenostandalone_amd import fields
+    from wenostandalone_amd.device import DeviceArray
+
+    wm.check(wm.lib.weno_init(0))
+    h = hip()
+    ns = m + 1 + 2 * buff
+    g = wm.make_grid(m, v=v, buff=buff)
+    f = fields.seeded_field(m, m, m, v, buff, seed=0)
+    vf = DeviceArray.from_host(f)
+    del f
+    coefs = []
+    for d in range(3):
+        devs = [DeviceArray.from_host(c) for c in fields.harness_coefficients(ns - 6, fields.DIR_OFFSETS[d])]
+        coefs.append((devs, wm.Coeffs(*[x.ptr for x in devs])))
+    L, R, rhs = DeviceArray((ns, ns, ns, v)), DeviceArray((ns, ns, ns, v)), DeviceArray((ns, ns, ns, v))
+    fx = wm.Flux(0.0, 1.0, float(m + 1))
+    pen = ns * ns * v * 8
+    fused_bytes = ((ns - 2) + (ns - 8)) * pen
+    two_bytes = ((ns - 2) + 2 * (ns - 6)) * pen + (2 * (ns - 6) + (ns - 8)) * pen
+    wm.check(wm.lib.weno_set_async(1))
+    stream = wm.lib.weno_get_stream()
+    out = {"workload": f"{m + 1}^3 x {v} FP64, WENO5-JS, Burgers (a=0, b=1) Rusanov flux divergence",
+           "fused_bytes_per_sweep": fused_bytes, "two_pass_bytes_per_sweep": two_bytes, "sweeps": {}}
+    for d, ax in enumerate("xyz"):
+        def fused():
+            wm.check(wm.lib.weno_reconstruct_flux(C.byref(g), 5, d, wm.SRC_FIELD, vf.vp, C.byref(coefs[d][1]),
+                                                  C.byref(fx), rhs.vp, 0))
+
+        def two():
+            wm.check(wm.lib.weno_reconstruct(C.byref(g), 5, d, wm.SRC_FIELD, wm.F64, vf.vp, C.byref(coefs[d][1]),
+                                             L.vp, R.vp, 0))
+            wm.check(wm.lib.weno_flux_divergence(C.byref(g), d, L.vp, R.vp, C.byref(fx), rhs.vp, 0))
+        tf = _time(h, stream, fused, reps=reps)
+        tt = _time(h, stream, two, reps=reps)
+        out["sweeps"][ax] = {"fused_us": tf * 1e3, "two_pass_us": tt * 1e3, "speedup": tt / tf,
+                             "fused_frac_of_hbm": fused_bytes / (tf * 1e-3) / (HBM_PEAK_GBS * 1e9)}
+        print(f"consumer {ax}: fused {tf * 1e3:.1f} us, two passes {tt * 1e3:.1f} us", file=sys.stderr, flush=True)
+    wm.check(wm.lib.weno_set_async(0))
+    for x in (vf, L, R, rhs):
+        x.free()
+    with open(path, "w") as fh:
+        json.dump(out, fh, indent=1)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -563,6 +619,8 @@ def main():
                     help="run only the suite's batched measurement (64 blocks, hipGraph) into this JSON file")
     ap.add_argument("--precision", default=None,
                     help="run only the suite's precision measurement (FP32/BF16 state) into this JSON file")
+    ap.add_argument("--consumer", default=None,
+                    help="time the fused reconstruction + flux divergence against two passes into this JSON file")
     ap.add_argument("--families", default=None,
                     help="time every kernel family against its roofline into this JSON file, then exit")
     ap.add_argument("--copy-yardstick", action="store_true",
@@ -581,6 +639,9 @@ def main():
 
     if args.families:
         families(args.families, m=args.m)
+        return
+    if args.consumer:
+        consumer(args.consumer, m=args.m)
         return
     for part in ("adjoint", "batched", "precision"):
         if getattr(args, part):

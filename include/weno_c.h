@@ -55,6 +55,7 @@ enum { WENO_F64 = 0, WENO_F32 = 1, WENO_BF16 = 2 };
 #define WENO_STREAMING 4   /* non-temporal stores */
 #define WENO_ONLY_L 8      /* write vL only (s_weno5_left_kernel, :1145) */
 #define WENO_ONLY_R 16     /* write vR only (s_weno5_right_kernel, :1272) */
+#define WENO_ACCUMULATE 32 /* flux divergence: rhs += contribution instead of = */
 
 /* Grid and scheme parameters: namelist &grid_params (weno_standalone.f90:
  * 57-66, :117-118) plus `teno`, `mapped_weno` and `wenoz`, compile-time
@@ -162,6 +163,35 @@ int weno_mp_limit(const weno_grid* g, int dir, int src_kind, int dtype, const vo
  * WENO_E_INVALID. */
 int weno_reconstruct_vjp(const weno_grid* g, int order, int dir, int src_kind, const void* src,
                          const weno_coeffs* c, const void* gL, const void* gR, void* gv);
+
+/* ---- consumer step (no reference counterpart) ---------------------------
+ * The harness stops at vL/vR (:343, :363, :383); MFC's next step, its
+ * Riemann solver and flux divergence, pairs qL(j) with qR(j+1) at interface
+ * j+1/2.  Here, per variable, the scalar flux f(q) = a q + b q^2 / 2 with a
+ * Rusanov (local Lax-Friedrichs) interface flux
+ *   F(qL, qR) = (f(qL) + f(qR))/2 - max(|a + b qL|, |a + b qR|) (qR - qL)/2
+ * gives, for the cells j = 4 .. ns-5 whose two interfaces lie in the
+ * reconstructed core,
+ *   rhs(j) = -(F(vL(j), vR(j+1)) - F(vL(j-1), vR(j))) * inv_dx
+ * in the sweep's face layout (ns, nk, nl, v_size), FP64; other cells of rhs
+ * keep their values.  WENO_ACCUMULATE adds the contribution to rhs. */
+typedef struct weno_flux {
+    double a;       /* linear part of f */
+    double b;       /* quadratic part of f (Burgers: a = 0, b = 1) */
+    double inv_dx;  /* 1 / cell width along the sweep */
+} weno_flux;
+
+/* Reconstruction (weno_reconstruct's order, weight branch and WENO_FUSE_MP)
+ * and flux divergence in one pass: the faces stay on chip, so the sweep
+ * reads the state once and writes rhs once.  FP64 state; sweeps of up to
+ * 1024 core points.  flags: WENO_FUSE_MP, WENO_ACCUMULATE. */
+int weno_reconstruct_flux(const weno_grid* g, int order, int dir, int src_kind, const void* src,
+                          const weno_coeffs* c, const weno_flux* fx, void* rhs, int flags);
+
+/* The same flux divergence over existing faces vL, vR (FP64, face layout):
+ * the second pass of an unfused reconstruction.  flags: WENO_ACCUMULATE. */
+int weno_flux_divergence(const weno_grid* g, int dir, const void* vL, const void* vR, const weno_flux* fx,
+                         void* rhs, int flags);
 
 /* ---- batched blocks, graphs, autotuning (no reference counterpart) ------ */
 /* One independent block (sub-domain): its own extents, variable count,

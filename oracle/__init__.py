@@ -49,6 +49,7 @@ def lib():
             "oracle_mp": [G, i, vp, vp, vp],
             "oracle_weno_vjp": [G, i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp],
             "oracle_set_teno_ratio_out": [vp],
+            "oracle_flux_divergence": [G, i, vp, vp, d, d, d, i, vp],
         }
         for name, args in protos.items():
             f = getattr(_lib, name)
@@ -157,6 +158,16 @@ def vjp(g: OGrid, order: int, d: int, v_rs: np.ndarray, coeffs, gL: np.ndarray, 
     lib().oracle_weno_vjp(C.byref(g), d, order, _p(np.asfortranarray(v_rs)), *[_p(c) for c in cs],
                           _p(np.asfortranarray(gL)), _p(np.asfortranarray(gR)), _p(gv))
     return gv
+
+
+def flux_divergence(g: OGrid, d: int, vL: np.ndarray, vR: np.ndarray, a: float, b: float, inv_dx: float,
+                    rhs: np.ndarray | None = None, accumulate: bool = False) -> np.ndarray:
+    """rhs of the Rusanov flux divergence over one sweep's faces (cells
+    4 .. ns-5); other cells keep rhs's values (zero when rhs is None)."""
+    out = np.zeros(vL.shape, order="F") if rhs is None else np.array(rhs, dtype=np.float64, order="F")
+    lib().oracle_flux_divergence(C.byref(g), d, _p(np.asfortranarray(vL)), _p(np.asfortranarray(vR)), a, b,
+                                 inv_dx, int(accumulate), _p(out))
+    return out
 
 
 def checksum(a: np.ndarray) -> float:

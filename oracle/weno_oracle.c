@@ -650,3 +650,32 @@ void oracle_weno_vjp(const oracle_grid* g, int dir, int order, const double* v_r
             }
 }
 #undef RS
+
+/* Rusanov interface flux of f(q) = a q + b q^2 / 2 (see weno_oracle.h) */
+static double rusanov_flux(double qL, double qR, double a, double b)
+{
+    double fL = a * qL + 0.5 * b * qL * qL;
+    double fR = a * qR + 0.5 * b * qR * qR;
+    double sL = fabs(a + b * qL), sR = fabs(a + b * qR);
+    double s = sL > sR ? sL : sR;
+    return 0.5 * (fL + fR) - 0.5 * s * (qR - qL);
+}
+
+void oracle_flux_divergence(const oracle_grid* g, int dir, const double* vL, const double* vR, double a,
+                            double b, double inv_dx, int accumulate, double* rhs)
+{
+    int ns, nk, nl;
+    oracle_dir_extents(g, dir, &ns, &nk, &nl);
+    for (int i = 0; i < g->v; ++i)
+        for (int l = 0; l < nl; ++l)
+#pragma omp parallel for schedule(static)
+            for (int k = 0; k < nk; ++k) {
+                size_t row = (size_t)ns * (k + (size_t)nk * (l + (size_t)nl * i));
+                for (int j = PAD + 1; j <= ns - 2 - PAD; ++j) {
+                    double Fp = rusanov_flux(vL[row + j], vR[row + j + 1], a, b);
+                    double Fm = rusanov_flux(vL[row + j - 1], vR[row + j], a, b);
+                    double r = -(Fp - Fm) * inv_dx;
+                    rhs[row + j] = accumulate ? rhs[row + j] + r : r;
+                }
+            }
+}

@@ -63,6 +63,17 @@ void oracle_weno7(const oracle_grid* g, int dir, const double* v_rs,
 /* A10: :1544-1615 */
 void oracle_mp(const oracle_grid* g, int dir, const double* v_rs, double* vL, double* vR);
 /* A6: :1617-1620 */
+/* Consumer of one sweep's faces (no reference counterpart: the harness stops
+ * at vL/vR; MFC's Riemann solvers pair qL(j) with qR(j+1) at interface
+ * j+1/2).  Per variable, scalar flux f(q) = a q + b q^2 / 2, Rusanov (local
+ * Lax-Friedrichs) interface flux
+ *   F(qL, qR) = (f(qL) + f(qR)) / 2 - max(|f'(qL)|, |f'(qR)|) (qR - qL) / 2,
+ * and rhs(j) = -(F(vL(j), vR(j+1)) - F(vL(j-1), vR(j))) * inv_dx for the
+ * cells j = 4 .. ns-5 whose two interfaces lie in the reconstructed core;
+ * accumulate != 0 adds to rhs.  Face layout (ns, nk, nl, v) for vL, vR, rhs. */
+void oracle_flux_divergence(const oracle_grid* g, int dir, const double* vL, const double* vR, double a,
+                            double b, double inv_dx, int accumulate, double* rhs);
+
 double oracle_checksum(const double* a, size_t n);
 
 /* test hook: per-cell TENO cut-off ratios of the next oracle_weno5/7 calls */

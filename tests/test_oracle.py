@@ -392,3 +392,81 @@ def test_teno_ratio_hook(order):
         assert np.array_equal(a / s[..., None], core)
     else:
         assert np.all(total <= 1.0)
+
+
+# ---- consumer step (no reference counterpart): oracle_flux_divergence ------
+def _faces(m=9, n=7, p=6, v=2, seed=5):
+    import oracle as O
+    from wenostandalone_amd import fields
+    og = O.grid(m, n, p, v=v, buff=6)
+    f = fields.seeded_field(m, n, p, v, 6, seed=seed)
+    out = []
+    for d in range(3):
+        ns = O.dir_extents(og, d)[0]
+        cs = fields.harness_coefficients(ns - 6, fields.DIR_OFFSETS[d])
+        cL, cR = O.order1_copy(og, d, f)
+        eL, eR = O.weno(og, 5, d, O.reshape(og, d, f), cs, cL, cR)
+        out.append((eL, eR))
+    return og, out
+
+
+@pytest.mark.parametrize("a", [1.7, -0.6])
+def test_flux_linear_advection_is_upwind(a):
+    """b = 0: the Rusanov flux is the upwind flux, F = a qL for a > 0 and
+    F = a qR for a < 0, so rhs(j) = -a (vL(j) - vL(j-1)) / dx or
+    -a (vR(j+1) - vR(j)) / dx -- which pins which face of which cell meets
+    at an interface."""
+    import oracle as O
+    og, faces = _faces()
+    for d, (vL, vR) in enumerate(faces):
+        rhs = O.flux_divergence(og, d, vL, vR, a, 0.0, 4.0)
+        ns = vL.shape[0]
+        if a > 0:
+            exp = -a * (vL[4:ns - 4] - vL[3:ns - 5]) * 4.0
+        else:
+            exp = -a * (vR[5:ns - 3] - vR[4:ns - 4]) * 4.0
+        scale = 8.0 * abs(a) * np.maximum(np.abs(vL[3:ns - 4]).max(), np.abs(vR[3:ns - 3]).max())
+        assert np.max(np.abs(rhs[4:ns - 4] - exp)) <= 1e-15 * scale, d
+        assert not rhs[:4].any() and not rhs[ns - 4:].any()
+
+
+def test_flux_burgers_constant_state_is_steady():
+    import oracle as O
+    og, faces = _faces()
+    for d, (vL, _) in enumerate(faces):
+        c = np.full(vL.shape, 1.375, order="F")
+        rhs = O.flux_divergence(og, d, c, c, 0.0, 1.0, 3.0)
+        assert not rhs.any()
+
+
+def test_flux_divergence_telescopes():
+    """Summed over a pencil, the divergence leaves the two boundary
+    interfaces: sum_j rhs(j) dx = -(F(ns-5 + 1/2) - F(4 - 1/2)), with the
+    interfaces j+1/2 = (vL(j), vR(j+1)).  A flux formed from any other pair
+    of faces would not cancel between neighbouring cells."""
+    import oracle as O
+    og, faces = _faces()
+    a, b, idx = 0.3, 0.8, 2.5
+
+    def F(qL, qR):
+        fl, fr = a * qL + 0.5 * b * qL * qL, a * qR + 0.5 * b * qR * qR
+        s = np.maximum(np.abs(a + b * qL), np.abs(a + b * qR))
+        return 0.5 * (fl + fr) - 0.5 * s * (qR - qL)
+    for d, (vL, vR) in enumerate(faces):
+        ns = vL.shape[0]
+        rhs = O.flux_divergence(og, d, vL, vR, a, b, idx)
+        total = rhs[4:ns - 4].sum(axis=0) / idx
+        exp = -(F(vL[ns - 5], vR[ns - 4]) - F(vL[3], vR[4]))
+        assert np.max(np.abs(total - exp)) <= 1e-12 * max(1.0, np.abs(exp).max()), d
+
+
+def test_flux_accumulate_and_untouched_cells():
+    import oracle as O
+    og, faces = _faces()
+    vL, vR = faces[1]
+    base = np.full(vL.shape, -2.0, order="F")
+    once = O.flux_divergence(og, 1, vL, vR, 0.5, 1.0, 1.0, rhs=base)
+    twice = O.flux_divergence(og, 1, vL, vR, 0.5, 1.0, 1.0, rhs=once, accumulate=True)
+    ns = vL.shape[0]
+    assert np.all(once[:4] == -2.0) and np.all(once[ns - 4:] == -2.0)
+    assert np.allclose(twice[4:ns - 4], 2.0 * once[4:ns - 4], rtol=1e-14, atol=0)

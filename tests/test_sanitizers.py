@@ -96,6 +96,13 @@ assert lib.weno_reconstruct(C.byref(g), 5, 0, 0, 7, None, C.byref(cf), None, Non
 assert lib.weno_reconstruct(C.byref(g), 5, 0, 0, 0, None, C.byref(cf), None, None, 1 << 9) == -1
 assert lib.weno_reconstruct(C.byref(g), 5, 0, 0, 0, None, C.byref(cf), None, None, 8 | 16) == -1
 assert lib.weno_reconstruct_vjp(C.byref(g), 3, 0, 0, None, C.byref(cf), None, None, None) == -1
+fxs = L.Flux(1.0, 0.0, 1.0)
+assert lib.weno_reconstruct_flux(C.byref(g), 4, 0, 0, None, C.byref(cf), C.byref(fxs), None, 0) == -1
+assert lib.weno_reconstruct_flux(C.byref(g), 5, 0, 0, None, C.byref(cf), None, None, 0) == -1
+assert lib.weno_reconstruct_flux(C.byref(L.make_grid(1100, 2, 2, v=1)), 5, 0, 0, None, C.byref(cf), C.byref(fxs),
+                                 None, 0) == -1
+assert lib.weno_flux_divergence(C.byref(g), 0, None, None, None, None, 0) == -1
+assert lib.weno_flux_divergence(C.byref(g), 0, None, None, C.byref(fxs), None, 1 << 9) == -1
 blocks = (L.Block * 300)()
 assert lib.weno_batched_reconstruct_noalloc(blocks, 300, 0, 2) == -1
 assert lib.weno_batched_reconstruct(blocks, 0, 0, 2) == -1

@@ -16,7 +16,7 @@ WENO_OK = 0
 DIR_X, DIR_Y, DIR_Z = 0, 1, 2
 SRC_FIELD, SRC_RESHAPED = 0, 1
 F64, F32, BF16 = 0, 1, 2
-FUSE_MP, FUSE_COPY, STREAMING, ONLY_L, ONLY_R = 1, 2, 4, 8, 16
+FUSE_MP, FUSE_COPY, STREAMING, ONLY_L, ONLY_R, ACCUMULATE = 1, 2, 4, 8, 16, 32
 
 
 class WenoError(RuntimeError):
@@ -45,6 +45,12 @@ class Coeffs(C.Structure):
 
     _fields_ = [("polyL", C.c_void_p), ("polyR", C.c_void_p), ("dL", C.c_void_p),
                 ("dR", C.c_void_p), ("beta", C.c_void_p)]
+
+
+class Flux(C.Structure):
+    """weno_flux: f(q) = a q + b q^2/2 and 1/dx of the consumer step."""
+
+    _fields_ = [("a", C.c_double), ("b", C.c_double), ("inv_dx", C.c_double)]
 
 
 class Block(C.Structure):
@@ -98,6 +104,8 @@ PROTOTYPES = {
     "weno_set_tile": (_i, [_i, _i, _i, _i, _i, _i, _i]),
     "weno_reconstruct_vjp": (_i, [_G, _i, _i, _i, _vp, _CF, _vp, _vp, _vp]),
     "weno_stream_copy": (_i, [_vp, _vp, _sz]),
+    "weno_reconstruct_flux": (_i, [_G, _i, _i, _i, _vp, _CF, C.POINTER(Flux), _vp, _i]),
+    "weno_flux_divergence": (_i, [_G, _i, _vp, _vp, C.POINTER(Flux), _vp, _i]),
     "weno_batched_reconstruct": (_i, [_vp, _i, _i, _i]),
     "weno_batched_reconstruct_noalloc": (_i, [_vp, _i, _i, _i]),
     "weno_batched_graph_create": (_i, [_vp, _i, _i, C.POINTER(_vp)]),
