@@ -38,8 +38,8 @@ HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
 # `bench.py --record-traffic FETCH_CSV WRITE_CSV`, used only while the kernel
 # sources hash the same as when it was measured
 TRAFFIC_RECORD = os.path.join(ROOT, "profiles", "sweep_weno5_fp64_200cube_traffic.json")
-KERNEL_SOURCES = ["weno_sweep5.hip", "weno_kernels.hip", "weno_device.h", "weno_internal.h", "weno_capi.hip",
-                  "weno_tune.hip"]
+KERNEL_SOURCES = ["weno_sweep5.hip", "weno_cell5.h", "weno_kernels.hip", "weno_device.h", "weno_internal.h",
+                  "weno_capi.hip", "weno_tune.hip"]
 SWEEP_KERNEL = "sweep5_kernel<"
 
 
@@ -68,10 +68,12 @@ def record_traffic(fetch_csv: str, write_csv: str, m: int = 199, v: int = 8) -> 
     fr, wr = rows(fetch_csv), rows(write_csv)
     cf = [x for k, x in fr if "stream_copy" in k]
     cw = [x for k, x in wr if "stream_copy" in k]
-    sf = [x for k, x in fr if SWEEP_KERNEL in k]
-    sw = [x for k, x in wr if SWEEP_KERNEL in k]
+    # the timed FP64 x/y/z sweeps come first (the precision leg, if run,
+    # launches the same kernels afterwards)
+    sf = [x for k, x in fr if SWEEP_KERNEL + "double" in k][:3]
+    sw = [x for k, x in wr if SWEEP_KERNEL + "double" in k][:3]
     if len(cf) != 1 or len(cw) != 1 or len(sf) != 3 or len(sw) != 3:
-        raise SystemExit("expected one stream copy and three sweeps in each counter file")
+        raise SystemExit("expected one stream copy and three FP64 sweeps in each counter file")
     kf, kw = copy_kib / cf[0], copy_kib / cw[0]
     per = [(a * kf + b * kw) * 1024 for a, b in zip(sf, sw)]
     rec = {"kernel_source_sha256": kernel_source_hash(), "traffic_bytes_per_launch": float(np.mean(per)),
