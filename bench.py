@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
 # `bench.py --record-traffic FETCH_CSV WRITE_CSV`, used only while the kernel
 # sources hash the same as when it was measured
 TRAFFIC_RECORD = os.path.join(ROOT, "profiles", "sweep_weno5_fp64_200cube_traffic.json")
-KERNEL_SOURCES = ["weno_sweep5.hip", "weno_cell5.h", "weno_kernels.hip", "weno_device.h", "weno_internal.h",
+KERNEL_SOURCES = ["weno_sweep5.hip", "weno_cells.h", "weno_kernels.hip", "weno_device.h", "weno_internal.h",
                   "weno_capi.hip", "weno_tune.hip"]
 SWEEP_KERNEL = "sweep5_kernel<"
 

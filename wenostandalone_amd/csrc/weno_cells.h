@@ -1,16 +1,19 @@
-// The WENO5-JS face pair of one cell in the shared-reciprocal form used by
-// the sweep kernels (the reference's s_weno5 JS branch,
-// inputs/weno_standalone.f90:477-1417):
+// The WENO5-JS and WENO7-JS face pairs of one cell in the shared-reciprocal
+// form used by the sweep kernels (the reference's JS branches of s_weno5,
+// inputs/weno_standalone.f90:477-1417, and s_weno7, :1419-1542).  WENO5:
 //     D_i = v(i+1) - v(i),  S_i = D_i^2,  X_i = D_i D_(i-1)
 //     beta_r = B_r0 S + B_r1 X + B_r2 S' + eps              (3 FMA each)
 //     u_r    = prod_{s != r} beta_s^2                     (6 multiplies)
 //     g_r    = (d_r a_r) D + (d_r b_r) D'                 (p_r = v_j + a_r D + b_r D')
 //     out    = v_j + (sum_r u_r g_r) / (sum_r d_r u_r)
 // which is  sum_r w_r p_r,  w_r = (d_r/beta_r^2)/sum_s(...),  regrouped so
-// that both sides share one reciprocal of (sum_L)(sum_R).  cell5 reports
-// when that product leaves the normal range; the caller then redoes the cell
-// with cell5_slow (per-stencil IEEE divisions, which also carry NaN and
-// infinity through as the reference does).
+// that both sides share one reciprocal of (sum_L)(sum_R).  WENO7 is the same
+// with four stencils, three differences per candidate and six terms per beta
+// (the 15 products D_i^2, D_i D_(i-1), D_i D_(i-2) shared by all four).
+// cell5/cell7 report when the shared product leaves the normal range; the
+// caller then redoes the cell with cell5_slow/cell7_slow (per-stencil IEEE
+// divisions, which also carry NaN and infinity through as the reference
+// does).
 #pragma once
 
 #include "weno_device.h"
@@ -104,6 +107,111 @@ __device__ __noinline__ void cell5_slow(double v0, double v1, double v2, double 
                                         double eps, double& oL, double& oR)
 {
     cell5_slow_inl(v0, v1, v2, v3, v4, k, eps, oL, oR);
+}
+
+// WENO7: stencil q (0..3) of the reference's pairing (:1457-1473) uses the
+// poly row 3-q, the differences D_(q+2), D_(q+1), D_q of the 7-point window
+// (D_i = v(i+1) - v(i)), beta(q) and d(q).
+struct Coef7 {
+    double eL[4][3], eR[4][3];   // d_q * poly(3-q, 0..2)
+    double b[4][6];              // beta_coef(j, 3-q, 0..5)
+    double dL[4], dR[4];
+};
+
+__device__ __forceinline__ void load_coef7(const Coefs& c, int jc, Coef7& k)
+{
+    const int nc = c.nc;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int r = 3 - q;
+        const double dl = c.dL[q + 5 * jc], dr = c.dR[q + 5 * jc];
+        k.dL[q] = dl;
+        k.dR[q] = dr;
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            k.eL[q][s] = dl * c.polyL[jc + nc * (r + 5 * s)];
+            k.eR[q][s] = dr * c.polyR[jc + nc * (r + 5 * s)];
+        }
+#pragma unroll
+        for (int s = 0; s < 6; ++s) k.b[q][s] = c.beta[jc + nc * (r + 5 * s)];
+    }
+}
+
+// beta(q) from the shared products: the reference's six terms
+// (:1462-1472) for stencil row 3-q, whose window starts at D_q.
+__device__ __forceinline__ void betas7(const double (&D)[6], const Coef7& k, double eps, double (&b)[4])
+{
+    double sq[6], n1[5], n2[4];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) sq[i] = D[i] * D[i];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) n1[i] = D[i + 1] * D[i];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) n2[i] = D[i + 2] * D[i];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        b[q] = fma(k.b[q][0], sq[q + 2],
+                   fma(k.b[q][1], n1[q + 1],
+                       fma(k.b[q][2], n2[q], fma(k.b[q][3], sq[q + 1], fma(k.b[q][4], n1[q], fma(k.b[q][5], sq[q], eps))))));
+}
+
+template <int NEWTON>
+__device__ __forceinline__ bool cell7(const double (&v)[7], const Coef7& k, double eps, double& oL, double& oR)
+{
+    double D[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) D[i] = v[i + 1] - v[i];
+    double b[4];
+    betas7(D, k, eps, b);
+    const double q0 = b[0] * b[0], q1 = b[1] * b[1], q2 = b[2] * b[2], q3 = b[3] * b[3];
+    const double q01 = q0 * q1, q23 = q2 * q3;
+    const double u[4] = {q1 * q23, q0 * q23, q01 * q3, q01 * q2};   // prod_{s != q} beta_s^2
+    double NL = 0.0, NR = 0.0, SL = 0.0, SR = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double gL = fma(k.eL[q][0], D[q + 2], fma(k.eL[q][1], D[q + 1], k.eL[q][2] * D[q]));
+        const double gR = fma(k.eR[q][0], D[q + 2], fma(k.eR[q][1], D[q + 1], k.eR[q][2] * D[q]));
+        NL = fma(u[q], gL, NL);
+        NR = fma(u[q], gR, NR);
+        SL = fma(k.dL[q], u[q], SL);
+        SR = fma(k.dR[q], u[q], SR);
+    }
+    const double T = SL * SR;
+    double r = __builtin_amdgcn_rcp(T);
+#pragma unroll
+    for (int it = 0; it < NEWTON; ++it) {
+        const double e = fma(-T, r, 1.0);
+        r = fma(r, e, r);
+    }
+    oL = fma(NL, SR * r, v[3]);
+    oR = fma(NR, SL * r, v[3]);
+    return (static_cast<int>(pos_normal(T)) & static_cast<int>(pos_normal(r))) != 0;
+}
+
+__device__ __forceinline__ void cell7_slow_inl(const double (&v)[7], const Coef7& k, double eps, double& oL,
+                                               double& oR)
+{
+    double D[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) D[i] = v[i + 1] - v[i];
+    double b[4];
+    betas7(D, k, eps, b);
+    double NL = 0.0, NR = 0.0, SL = 0.0, SR = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double y = 1.0 / (b[q] * b[q]);
+        NL += fma(k.eL[q][0], D[q + 2], fma(k.eL[q][1], D[q + 1], k.eL[q][2] * D[q])) * y;
+        NR += fma(k.eR[q][0], D[q + 2], fma(k.eR[q][1], D[q + 1], k.eR[q][2] * D[q])) * y;
+        SL += k.dL[q] * y;
+        SR += k.dR[q] * y;
+    }
+    oL = v[3] + NL / SL;
+    oR = v[3] + NR / SR;
+}
+
+__device__ __noinline__ void cell7_slow(const double (&v)[7], const Coef7& k, double eps, double& oL, double& oR)
+{
+    cell7_slow_inl(v, k, eps, oL, oR);
 }
 
 }  // namespace

@@ -21,7 +21,7 @@
 // back and writes rhs.  weno_flux_divergence is that separate pass, for
 // faces that already exist (faces_flux_kernel).
 #include "weno_device.h"
-#include "weno_cell5.h"
+#include "weno_cells.h"
 #include "weno_capi_internal.h"
 
 #include <algorithm>

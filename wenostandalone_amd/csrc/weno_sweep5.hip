@@ -47,12 +47,13 @@
 // filled 128-byte lines back at about 60% of the rate of full ones.
 //
 // Per face pair the JS weights take the shared-reciprocal form of cell5
-// (weno_cell5.h): both sides share one reciprocal, and lanes whose product
+// (weno_cells.h): both sides share one reciprocal, and lanes whose product
 // leaves the normal range redo the cell with per-stencil IEEE divisions.
 #include "weno_device.h"
-#include "weno_cell5.h"
+#include "weno_cells.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace wm {
 
@@ -206,10 +207,10 @@ __device__ __forceinline__ long long out_base(const TileArgs& a, int iv, int pa,
 //        of a tile are consecutive along x, loaded as 128-byte row segments
 //        (one pencil per lane, 4 rows per wave instruction) and transposed
 //        into LDS.
-template <typename TS, int LAY, bool COPY, bool PAIR, bool MP>
-__global__ void __attribute__((amdgpu_flat_work_group_size(64, kThreads5), amdgpu_waves_per_eu(4)))
-sweep5_kernel(const TileArgs a)
+template <int ORDER, typename TS, int LAY, bool COPY, bool PAIR, bool MP>
+__device__ __forceinline__ void sweep_body(const TileArgs& a)
 {
+    constexpr int RS = ORDER / 2;   // stencil radius
     using TO = typename Store5<TS>::face;
     using Piece = typename Store5<TS>::piece;
     using Raw = typename Store5<TS>::raw;
@@ -252,7 +253,7 @@ sweep5_kernel(const TileArgs a)
     const int jo1 = min(a.out_len, jo0 + a.chunk);
     const int c0 = max(jo0, a.j_beg), c1 = min(jo1 - 1, a.j_end);
     if (!COPY && c0 > c1) return;   // nothing to reconstruct in this chunk
-    int lo = c0 + a.J0f - 2, hi = c1 + a.J0f + 2;
+    int lo = c0 + a.J0f - RS, hi = c1 + a.J0f + RS;
     if (COPY) {
         lo = c0 <= c1 ? min(jo0 + a.J0f, lo) : jo0 + a.J0f;
         hi = c0 <= c1 ? max(jo1 - 1 + a.J0f, hi) : jo1 - 1 + a.J0f;
@@ -403,13 +404,16 @@ sweep5_kernel(const TileArgs a)
     const bool halo = active && !core;
     const int col = j + a.J0f - Lb;
     // halo lanes run the stencil on a clamped window and keep their value
-    const int colc = min(max(col, 2), nrow - 3);
+    const int colc = ORDER == 5 ? min(max(col, 2), nrow - 3) : min(max(col, RS), nrow - 1 - RS);
     const int hpos = j < a.j_beg ? j - (a.j_beg - 3) : j - (a.j_end + 1) + 3;
-    Coef5 k5;
+    std::conditional_t<ORDER == 7, Coef7, Coef5> k5;
     {
         int jc = j - 3;
         jc = jc < 0 ? 0 : (jc > a.c.nc - 1 ? a.c.nc - 1 : jc);
-        load_coef5(a.c, jc, k5);
+        if constexpr (ORDER == 7)
+            load_coef7(a.c, jc, k5);
+        else
+            load_coef5(a.c, jc, k5);
     }
     const double eps = a.eps;
     const unsigned joff = static_cast<unsigned>(j * sizeof(TO));
@@ -427,30 +431,68 @@ sweep5_kernel(const TileArgs a)
         const double* const cur = ((k & 1) ? buf1 : buf0) + gq * ldj;
         if (active) {
             bool allok = true;
+            if constexpr (ORDER == 5) {
 #pragma unroll 1
-            for (int qq = 0; qq < kNPG; ++qq) {
-                const int q = 2 * qq + gq;
-                const double* row = cur + 2 * qq * ldj + colc;
-                const double v0 = row[-2], v1 = row[-1], v2 = row[0], v3 = row[1], v4 = row[2];
-                double oL, oR;
-                allok &= cell5<NEWTON>(v0, v1, v2, v3, v4, k5, eps, oL, oR);
-                if constexpr (MP) {
-                    // every lane limits (halo lanes replace their value below)
-                    const double w[5] = {v0, v1, v2, v3, v4};
-                    mp_limit<double>(w, oL, oR);
+                for (int qq = 0; qq < kNPG; ++qq) {
+                    const int q = 2 * qq + gq;
+                    const double* row = cur + 2 * qq * ldj + colc;
+                    const double v0 = row[-2], v1 = row[-1], v2 = row[0], v3 = row[1], v4 = row[2];
+                    double oL, oR;
+                    allok &= cell5<NEWTON>(v0, v1, v2, v3, v4, k5, eps, oL, oR);
+                    if constexpr (MP) {
+                        // every lane limits (halo lanes replace their value below)
+                        const double w[5] = {v0, v1, v2, v3, v4};
+                        mp_limit<double>(w, oL, oR);
+                    }
+                    TO wL = static_cast<TO>(oL), wR = static_cast<TO>(oR);
+                    if (halo) {
+                        if constexpr (COPY) {
+                            wL = wR = static_cast<TO>(row[col - colc]);
+                        } else {
+                            wL = s_halo[slot][q][hpos][0];
+                            wR = s_halo[slot][q][hpos][1];
+                        }
+                    }
+                    const unsigned off = s_off[slot][q] + joff;
+                    store_face(wL, rL, off);
+                    store_face(wR, rR, off);
                 }
-                TO wL = static_cast<TO>(oL), wR = static_cast<TO>(oR);
-                if (halo) {
-                    if constexpr (COPY) {
-                        wL = wR = static_cast<TO>(row[col - colc]);
-                    } else {
-                        wL = s_halo[slot][q][hpos][0];
-                        wR = s_halo[slot][q][hpos][1];
+            } else {
+                // WENO7 lanes take two pencils per step: two independent
+                // dependency chains at 2 waves per SIMD
+#pragma unroll 1
+                for (int qq0 = 0; qq0 < kNPG; qq0 += 2) {
+                    double oL[2], oR[2];
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) {
+                        const double* row = cur + 2 * (qq0 + e) * ldj + colc;
+                        double w[7];
+#pragma unroll
+                        for (int u = 0; u < 7; ++u) w[u] = row[u - 3];
+                        allok &= cell7<NEWTON>(w, k5, eps, oL[e], oR[e]);
+                        if constexpr (MP) {
+                            const double wm[5] = {w[1], w[2], w[3], w[4], w[5]};
+                            mp_limit<double>(wm, oL[e], oR[e]);
+                        }
+                    }
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) {
+                        const int q = 2 * (qq0 + e) + gq;
+                        const double* row = cur + 2 * (qq0 + e) * ldj + colc;
+                        TO wL = static_cast<TO>(oL[e]), wR = static_cast<TO>(oR[e]);
+                        if (halo) {
+                            if constexpr (COPY) {
+                                wL = wR = static_cast<TO>(row[col - colc]);
+                            } else {
+                                wL = s_halo[slot][q][hpos][0];
+                                wR = s_halo[slot][q][hpos][1];
+                            }
+                        }
+                        const unsigned off = s_off[slot][q] + joff;
+                        store_face(wL, rL, off);
+                        store_face(wR, rR, off);
                     }
                 }
-                const unsigned off = s_off[slot][q] + joff;
-                store_face(wL, rL, off);
-                store_face(wR, rR, off);
             }
             if (!allok && core) {
 #pragma unroll 1
@@ -458,7 +500,16 @@ sweep5_kernel(const TileArgs a)
                     const int q = 2 * qq + gq;
                     const double* row = cur + 2 * qq * ldj + colc;
                     double oL, oR;
-                    cell5_slow(row[-2], row[-1], row[0], row[1], row[2], k5, eps, oL, oR);
+                    if constexpr (ORDER == 7) {
+                        // inline: an out-of-line call would keep the 56
+                        // coefficients in scratch and reload them every tile
+                        double w[7];
+#pragma unroll
+                        for (int u = 0; u < 7; ++u) w[u] = row[u - 3];
+                        cell7_slow_inl(w, k5, eps, oL, oR);
+                    } else {
+                        cell5_slow(row[-2], row[-1], row[0], row[1], row[2], k5, eps, oL, oR);
+                    }
                     if constexpr (MP) {
                         const double w[5] = {row[-2], row[-1], row[0], row[1], row[2]};
                         mp_limit<double>(w, oL, oR);
@@ -473,7 +524,25 @@ sweep5_kernel(const TileArgs a)
     }
 }
 
-// Host side: decides whether a WENO5-JS launch runs on this kernel and sets
+// The WENO5-JS kernel: 128 VGPRs, 4 waves per SIMD (two workgroups per CU).
+template <typename TS, int LAY, bool COPY, bool PAIR, bool MP>
+__global__ void __attribute__((amdgpu_flat_work_group_size(64, kThreads5), amdgpu_waves_per_eu(4)))
+sweep5_kernel(const TileArgs a)
+{
+    sweep_body<5, TS, LAY, COPY, PAIR, MP>(a);
+}
+
+// The WENO7-JS kernel: the 56 coefficients of a lane take 112 VGPRs, so it
+// runs at 2 waves per SIMD (one workgroup per CU).
+template <typename TS, int LAY, bool COPY, bool PAIR, bool MP>
+__global__ void __attribute__((amdgpu_flat_work_group_size(64, kThreads5), amdgpu_waves_per_eu(2)))
+sweep7_kernel(const TileArgs a)
+{
+    sweep_body<7, TS, LAY, COPY, PAIR, MP>(a);
+}
+
+// Host side: decides whether a WENO5-JS or WENO7-JS launch runs on these
+// kernels and sets
 // its tile shape.  Returns the layout (0 or 1), or -1 when the general sweep
 // kernel must be used (other orders/modes, a fused limiter range other than
 // the reconstructed range, one-sided output, rows shorter than 96 outputs,
@@ -481,7 +550,8 @@ sweep5_kernel(const TileArgs a)
 // offsets).
 int plan_sweep5(TileArgs& a, Launch& l, const void* src)
 {
-    if (l.op != OP_WENO || l.order != 5 || l.mode != MODE_JS || a.sides != 3) return -1;
+    if (l.op != OP_WENO || (l.order != 5 && l.order != 7) || l.mode != MODE_JS || a.sides != 3) return -1;
+    const int RS = l.order / 2;
     if (l.mp && (a.mp_beg != a.j_beg || a.mp_end != a.j_end)) return -1;
     int in_b, out_b;
     switch (l.dtype) {
@@ -519,7 +589,7 @@ int plan_sweep5(TileArgs& a, Launch& l, const void* src)
             if (jo0 >= jo1) break;
             const int c0 = std::max(jo0, a.j_beg), c1 = std::min(jo1 - 1, a.j_end);
             if (c0 > c1 && !a.copy_halo) continue;
-            int lo = c0 + a.J0f - 2, hi = c1 + a.J0f + 2;
+            int lo = c0 + a.J0f - RS, hi = c1 + a.J0f + RS;
             if (a.copy_halo) {
                 lo = c0 <= c1 ? std::min(jo0 + a.J0f, lo) : jo0 + a.J0f;
                 hi = c0 <= c1 ? std::max(jo1 - 1 + a.J0f, hi) : jo1 - 1 + a.J0f;
@@ -556,28 +626,42 @@ int plan_sweep5(TileArgs& a, Launch& l, const void* src)
 
 namespace {
 typedef void (*Sweep5Fn)(const TileArgs);
-template <typename TS, bool MP>
+template <int ORDER, typename TS, int LAY, bool COPY, bool PAIR, bool MP>
+Sweep5Fn kernel_of()
+{
+    if constexpr (ORDER == 7)
+        return sweep7_kernel<TS, LAY, COPY, PAIR, MP>;
+    else
+        return sweep5_kernel<TS, LAY, COPY, PAIR, MP>;
+}
+template <int ORDER, typename TS, bool MP>
 Sweep5Fn pick5(int lay, bool pair, bool copy)
 {
-    return lay == 0 ? (copy ? sweep5_kernel<TS, 0, true, false, MP> : sweep5_kernel<TS, 0, false, false, MP>)
-           : pair   ? (copy ? sweep5_kernel<TS, 1, true, true, MP> : sweep5_kernel<TS, 1, false, true, MP>)
-                    : (copy ? sweep5_kernel<TS, 1, true, false, MP> : sweep5_kernel<TS, 1, false, false, MP>);
+    return lay == 0 ? (copy ? kernel_of<ORDER, TS, 0, true, false, MP>() : kernel_of<ORDER, TS, 0, false, false, MP>())
+           : pair   ? (copy ? kernel_of<ORDER, TS, 1, true, true, MP>() : kernel_of<ORDER, TS, 1, false, true, MP>())
+                    : (copy ? kernel_of<ORDER, TS, 1, true, false, MP>() : kernel_of<ORDER, TS, 1, false, false, MP>());
 }
-template <typename TS>
+template <int ORDER, typename TS>
 Sweep5Fn pick5(int lay, bool pair, bool copy, bool mp)
 {
-    return mp ? pick5<TS, true>(lay, pair, copy) : pick5<TS, false>(lay, pair, copy);
+    return mp ? pick5<ORDER, TS, true>(lay, pair, copy) : pick5<ORDER, TS, false>(lay, pair, copy);
+}
+template <int ORDER>
+Sweep5Fn pick5(int dtype, int lay, bool pair, bool copy, bool mp)
+{
+    return dtype == DT_F64   ? pick5<ORDER, double>(lay, pair, copy, mp)
+           : dtype == DT_F32 ? pick5<ORDER, float>(lay, pair, copy, mp)
+                             : pick5<ORDER, bf16_t>(lay, pair, copy, mp);
 }
 }  // namespace
 
-hipError_t launch_sweep5(TileArgs a, int lay, int dtype, int mp, hipStream_t st)
+hipError_t launch_sweep5(TileArgs a, int order, int lay, int dtype, int mp, hipStream_t st)
 {
     if (a.threads != kThreads5 || 2 * a.jn > kConsumers || a.ldj > 0xffff) return hipErrorInvalidConfiguration;
     const size_t lds = 2 * static_cast<size_t>(kNP) * a.ldj * sizeof(double);
     const bool copy = a.copy_halo != 0, pair = a.pair_b != 0;
-    const Sweep5Fn kern = dtype == DT_F64   ? pick5<double>(lay, pair, copy, mp != 0)
-                          : dtype == DT_F32 ? pick5<float>(lay, pair, copy, mp != 0)
-                                            : pick5<bf16_t>(lay, pair, copy, mp != 0);
+    const Sweep5Fn kern = order == 7 ? pick5<7>(dtype, lay, pair, copy, mp != 0)
+                                     : pick5<5>(dtype, lay, pair, copy, mp != 0);
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
