@@ -405,41 +405,40 @@ __device__ __forceinline__ double wenoz_combine(const double (&b)[N], double tau
 }
 
 // ---- stencil terms: candidate polynomials and smoothness indicators --------
-// Each body is written in the reference's evaluation order (left-to-right
-// sums and products, :421-428, :529-550, :1450-1473) and instantiated twice:
-// with contraction (FMA) for the JS weights, which are well conditioned in
-// beta, and without (`fp contract(off)`: HIP's default contraction mode
-// honours the pragma) for the mapped, WENO-Z and TENO branches.  Their
-// weights take differences of betas (tau = |beta_a - beta_b|) or cut at a
-// threshold, so one-ulp differences in beta would be amplified; evaluated
-// exactly as the reference does, beta and tau are bit-identical to it.
-#define WM_TERMS3                                                                           \
-    const C d0 = v[2] - v[1], dm1 = v[1] - v[0];                                            \
+// Differences and candidates (DIFF/CAND) and smoothness indicators (BETA) of
+// each order, written in the reference's evaluation order (left-to-right
+// sums and products, :421-428, :529-550, :1450-1473).
+#define WM_DIFF3 const C d0 = v[2] - v[1], dm1 = v[1] - v[0];
+#define WM_CAND3                                                                            \
     pL[0] = v[1] + f.pl[0] * d0;                                                            \
     pL[1] = v[1] + f.pl[1] * dm1;                                                           \
     pR[0] = v[1] + f.pr[0] * d0;                                                            \
-    pR[1] = v[1] + f.pr[1] * dm1;                                                           \
+    pR[1] = v[1] + f.pr[1] * dm1;
+#define WM_BETA3                                                                            \
     const double D0 = d0, Dm1 = dm1;                                                        \
     b0 = B(0, 0) * D0 * D0 + eps;                                                           \
     b1 = B(1, 0) * Dm1 * Dm1 + eps;
-#define WM_TERMS5                                                                           \
+#define WM_DIFF5                                                                            \
     const C vj = v[2];                                                                      \
-    const C dp1 = v[4] - v[3], d0 = v[3] - v[2], dm1 = v[2] - v[1], dm2 = v[1] - v[0];      \
+    const C dp1 = v[4] - v[3], d0 = v[3] - v[2], dm1 = v[2] - v[1], dm2 = v[1] - v[0];
+#define WM_CAND5                                                                            \
     pL[0] = vj + f.pl[0][0] * dp1 + f.pl[0][1] * d0;                                        \
     pL[1] = vj + f.pl[1][0] * d0 + f.pl[1][1] * dm1;                                        \
     pL[2] = vj + f.pl[2][0] * dm1 + f.pl[2][1] * dm2;                                       \
     pR[0] = vj + f.pr[0][0] * dp1 + f.pr[0][1] * d0;                                        \
     pR[1] = vj + f.pr[1][0] * d0 + f.pr[1][1] * dm1;                                        \
-    pR[2] = vj + f.pr[2][0] * dm1 + f.pr[2][1] * dm2;                                       \
+    pR[2] = vj + f.pr[2][0] * dm1 + f.pr[2][1] * dm2;
+#define WM_BETA5                                                                            \
     const double Dp1 = dp1, D0 = d0, Dm1 = dm1, Dm2 = dm2;                                  \
     b0 = B(0, 0) * Dp1 * Dp1 + B(0, 1) * Dp1 * D0 + B(0, 2) * D0 * D0 + eps;                \
     b1 = B(1, 0) * D0 * D0 + B(1, 1) * D0 * Dm1 + B(1, 2) * Dm1 * Dm1 + eps;                \
     b2 = B(2, 0) * Dm1 * Dm1 + B(2, 1) * Dm1 * Dm2 + B(2, 2) * Dm2 * Dm2 + eps;
 // WENO7: poly(q)/beta(q) use coefficient row 3-q (:1457-1473)
-#define WM_TERMS7                                                                           \
+#define WM_DIFF7                                                                            \
     const C vj = v[3];                                                                      \
     const C d2 = v[6] - v[5], d1 = v[5] - v[4], d0 = v[4] - v[3];                           \
-    const C dm1 = v[3] - v[2], dm2 = v[2] - v[1], dm3 = v[1] - v[0];                        \
+    const C dm1 = v[3] - v[2], dm2 = v[2] - v[1], dm3 = v[1] - v[0];
+#define WM_CAND7                                                                            \
     pL[0] = vj + f.pl[3][0] * dm1 + f.pl[3][1] * dm2 + f.pl[3][2] * dm3;                    \
     pL[1] = vj + f.pl[2][0] * d0 + f.pl[2][1] * dm1 + f.pl[2][2] * dm2;                     \
     pL[2] = vj + f.pl[1][0] * d1 + f.pl[1][1] * d0 + f.pl[1][2] * dm1;                      \
@@ -447,7 +446,8 @@ __device__ __forceinline__ double wenoz_combine(const double (&b)[N], double tau
     pR[0] = vj + f.pr[3][0] * dm1 + f.pr[3][1] * dm2 + f.pr[3][2] * dm3;                    \
     pR[1] = vj + f.pr[2][0] * d0 + f.pr[2][1] * dm1 + f.pr[2][2] * dm2;                     \
     pR[2] = vj + f.pr[1][0] * d1 + f.pr[1][1] * d0 + f.pr[1][2] * dm1;                      \
-    pR[3] = vj + f.pr[0][0] * d2 + f.pr[0][1] * d1 + f.pr[0][2] * d0;                       \
+    pR[3] = vj + f.pr[0][0] * d2 + f.pr[0][1] * d1 + f.pr[0][2] * d0;
+#define WM_BETA7                                                                            \
     const double D2 = d2, D1 = d1, D0 = d0, Dm1 = dm1, Dm2 = dm2, Dm3 = dm3;                \
     b[3] = B(0, 0) * D2 * D2 + B(0, 1) * D2 * D1 + B(0, 2) * D2 * D0 + B(0, 3) * D1 * D1 +  \
            B(0, 4) * D1 * D0 + B(0, 5) * D0 * D0 + eps;                                     \
@@ -458,28 +458,44 @@ __device__ __forceinline__ double wenoz_combine(const double (&b)[N], double tau
     b[0] = B(3, 0) * Dm1 * Dm1 + B(3, 1) * Dm1 * Dm2 + B(3, 2) * Dm1 * Dm3 + B(3, 3) * Dm2 * Dm2 + \
            B(3, 4) * Dm2 * Dm3 + B(3, 5) * Dm3 * Dm3 + eps;
 
-#define WM_TERMS_FNS(O, ARGS, BODY)                                                         \
+// _fused: everything contracted (the JS weights, well conditioned in beta).
+// _exact: the smoothness indicators without contraction (`fp contract(off)`:
+// HIP's default contraction mode honours the pragma), for the mapped, WENO-Z
+// and TENO branches, whose weights take differences of betas
+// (tau = |beta_a - beta_b|) or cut at a threshold, so one-ulp differences in
+// beta would be amplified; evaluated exactly as the reference does, beta and
+// tau are bit-identical to it.  The candidates enter only the final weighted
+// sum, so they are contracted in both (three FMAs each).
+#define WM_TERMS_FNS(O, ARGS)                                                               \
     template <typename C, typename BT>                                                      \
     __device__ __forceinline__ void terms##O##_fused(const C* v, const Cf<O, C>& f, const BT& B, double eps, ARGS) \
     {                                                                                       \
-        BODY                                                                                \
+        WM_DIFF##O WM_CAND##O WM_BETA##O                                                    \
     }                                                                                       \
     template <typename C, typename BT>                                                      \
     __device__ __forceinline__ void terms##O##_exact(const C* v, const Cf<O, C>& f, const BT& B, double eps, ARGS) \
     {                                                                                       \
-        _Pragma("clang fp contract(off)")                                                   \
-        BODY                                                                                \
+        WM_DIFF##O WM_CAND##O                                                               \
+        {                                                                                   \
+            _Pragma("clang fp contract(off)")                                               \
+            WM_BETA##O                                                                      \
+        }                                                                                   \
     }
 #define WM_COMMA ,
-WM_TERMS_FNS(3, C (&pL)[2] WM_COMMA C (&pR)[2] WM_COMMA double& b0 WM_COMMA double& b1, WM_TERMS3)
-WM_TERMS_FNS(5, C (&pL)[3] WM_COMMA C (&pR)[3] WM_COMMA double& b0 WM_COMMA double& b1 WM_COMMA double& b2,
-             WM_TERMS5)
-WM_TERMS_FNS(7, C (&pL)[4] WM_COMMA C (&pR)[4] WM_COMMA double (&b)[4], WM_TERMS7)
+WM_TERMS_FNS(3, C (&pL)[2] WM_COMMA C (&pR)[2] WM_COMMA double& b0 WM_COMMA double& b1)
+WM_TERMS_FNS(5, C (&pL)[3] WM_COMMA C (&pR)[3] WM_COMMA double& b0 WM_COMMA double& b1 WM_COMMA double& b2)
+WM_TERMS_FNS(7, C (&pL)[4] WM_COMMA C (&pR)[4] WM_COMMA double (&b)[4])
 #undef WM_TERMS_FNS
 #undef WM_COMMA
-#undef WM_TERMS3
-#undef WM_TERMS5
-#undef WM_TERMS7
+#undef WM_DIFF3
+#undef WM_CAND3
+#undef WM_BETA3
+#undef WM_DIFF5
+#undef WM_CAND5
+#undef WM_BETA5
+#undef WM_DIFF7
+#undef WM_CAND7
+#undef WM_BETA7
 
 template <bool EXACT, typename C, typename BT>
 __device__ __forceinline__ void terms3(const C* v, const Cf<3, C>& f, const BT& B, double eps, C (&pL)[2],

@@ -81,7 +81,7 @@ hipError_t launch_tile(const TileArgs& a, const Launch& l, hipStream_t st);
 // WENO5-JS producer/consumer kernel: planning (returns the layout, or -1 if
 // the launch stays on the general kernel) and launch.
 int plan_sweep5(TileArgs& a, Launch& l, const void* src);
-hipError_t launch_sweep5(TileArgs a, int order, int lay, int dtype, int mp, hipStream_t st);
+hipError_t launch_sweep5(TileArgs a, int order, int mode, int lay, int dtype, int mp, hipStream_t st);
 
 // LDS element size of the compute type for a storage dtype
 inline int lds_elem_bytes(int dtype) { return dtype == DT_F64 ? 8 : 4; }

@@ -387,7 +387,7 @@ int max_threads(int op, int order);
 hipError_t launch_tile(const TileArgs& a, const Launch& l, hipStream_t st)
 {
     if (a.tiles_per_var <= 0 || a.nv <= 0) return hipSuccess;
-    if (l.fast5 > 0) return launch_sweep5(a, l.order, l.fast5 - 1, l.dtype, l.mp, st);
+    if (l.fast5 > 0) return launch_sweep5(a, l.order, l.mode, l.fast5 - 1, l.dtype, l.mp, st);
     if (a.np > kMaxNP || a.slots > kMaxSlots || a.threads > max_threads(l.op, l.order) || a.threads < 64 ||
         a.ldj > 0xffff)
         return hipErrorInvalidConfiguration;

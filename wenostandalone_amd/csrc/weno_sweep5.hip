@@ -207,7 +207,15 @@ __device__ __forceinline__ long long out_base(const TileArgs& a, int iv, int pa,
 //        of a tile are consecutive along x, loaded as 128-byte row segments
 //        (one pencil per lane, 4 rows per wave instruction) and transposed
 //        into LDS.
-template <int ORDER, typename TS, int LAY, bool COPY, bool PAIR, bool MP>
+// WENO7 weight branches other than JS: the raw candidate and smoothness
+// coefficients and linear weights of the lane, in registers, for the
+// general per-family arithmetic (weno7 in weno_device.h).
+struct Gen7 {
+    Cf<7, double> f;
+    BetaRegs<7> b;
+};
+
+template <int ORDER, int MODE, typename TS, int LAY, bool COPY, bool PAIR, bool MP>
 __device__ __forceinline__ void sweep_body(const TileArgs& a)
 {
     constexpr int RS = ORDER / 2;   // stencil radius
@@ -406,14 +414,18 @@ __device__ __forceinline__ void sweep_body(const TileArgs& a)
     // halo lanes run the stencil on a clamped window and keep their value
     const int colc = ORDER == 5 ? min(max(col, 2), nrow - 3) : min(max(col, RS), nrow - 1 - RS);
     const int hpos = j < a.j_beg ? j - (a.j_beg - 3) : j - (a.j_end + 1) + 3;
-    std::conditional_t<ORDER == 7, Coef7, Coef5> k5;
+    std::conditional_t<ORDER == 7, std::conditional_t<MODE == MODE_JS, Coef7, Gen7>, Coef5> k5;
     {
         int jc = j - 3;
         jc = jc < 0 ? 0 : (jc > a.c.nc - 1 ? a.c.nc - 1 : jc);
-        if constexpr (ORDER == 7)
-            load_coef7(a.c, jc, k5);
-        else
+        if constexpr (ORDER == 5) {
             load_coef5(a.c, jc, k5);
+        } else if constexpr (MODE == MODE_JS) {
+            load_coef7(a.c, jc, k5);
+        } else {
+            load_cf(a.c, jc, k5.f);
+            k5.b.load(a.c, jc);
+        }
     }
     const double eps = a.eps;
     const unsigned joff = static_cast<unsigned>(j * sizeof(TO));
@@ -469,7 +481,10 @@ __device__ __forceinline__ void sweep_body(const TileArgs& a)
                         double w[7];
 #pragma unroll
                         for (int u = 0; u < 7; ++u) w[u] = row[u - 3];
-                        allok &= cell7<NEWTON>(w, k5, eps, oL[e], oR[e]);
+                        if constexpr (MODE == MODE_JS)
+                            allok &= cell7<NEWTON>(w, k5, eps, oL[e], oR[e]);
+                        else
+                            weno7<MODE, double>(w, k5.f, k5.b, eps, a.teno_ct, oL[e], oR[e]);
                         if constexpr (MP) {
                             const double wm[5] = {w[1], w[2], w[3], w[4], w[5]};
                             mp_limit<double>(wm, oL[e], oR[e]);
@@ -506,7 +521,7 @@ __device__ __forceinline__ void sweep_body(const TileArgs& a)
                         double w[7];
 #pragma unroll
                         for (int u = 0; u < 7; ++u) w[u] = row[u - 3];
-                        cell7_slow_inl(w, k5, eps, oL, oR);
+                        if constexpr (MODE == MODE_JS) cell7_slow_inl(w, k5, eps, oL, oR);
                     } else {
                         cell5_slow(row[-2], row[-1], row[0], row[1], row[2], k5, eps, oL, oR);
                     }
@@ -529,16 +544,18 @@ template <typename TS, int LAY, bool COPY, bool PAIR, bool MP>
 __global__ void __attribute__((amdgpu_flat_work_group_size(64, kThreads5), amdgpu_waves_per_eu(4)))
 sweep5_kernel(const TileArgs a)
 {
-    sweep_body<5, TS, LAY, COPY, PAIR, MP>(a);
+    sweep_body<5, MODE_JS, TS, LAY, COPY, PAIR, MP>(a);
 }
 
-// The WENO7-JS kernel: the 56 coefficients of a lane take 112 VGPRs, so it
-// runs at 2 waves per SIMD (one workgroup per CU).
-template <typename TS, int LAY, bool COPY, bool PAIR, bool MP>
+// The WENO7 kernel: the 56 coefficients of a lane take 112 VGPRs, so it
+// runs at 2 waves per SIMD (one workgroup per CU).  JS for every storage
+// type, with or without the fused limiter; the other weight branches for
+// FP64 state without it.
+template <int MODE, typename TS, int LAY, bool COPY, bool PAIR, bool MP>
 __global__ void __attribute__((amdgpu_flat_work_group_size(64, kThreads5), amdgpu_waves_per_eu(2)))
 sweep7_kernel(const TileArgs a)
 {
-    sweep_body<7, TS, LAY, COPY, PAIR, MP>(a);
+    sweep_body<7, MODE, TS, LAY, COPY, PAIR, MP>(a);
 }
 
 // Host side: decides whether a WENO5-JS or WENO7-JS launch runs on these
@@ -550,7 +567,13 @@ sweep7_kernel(const TileArgs a)
 // offsets).
 int plan_sweep5(TileArgs& a, Launch& l, const void* src)
 {
-    if (l.op != OP_WENO || (l.order != 5 && l.order != 7) || l.mode != MODE_JS || a.sides != 3) return -1;
+    if (l.op != OP_WENO || (l.order != 5 && l.order != 7) || a.sides != 3) return -1;
+    if (l.mode != MODE_JS) {
+        // WENO7's other weight branches: FP64 state, no fused limiter
+        if (l.order != 7 || l.dtype != DT_F64 || l.mp ||
+            (l.mode != MODE_TENO && l.mode != MODE_WENOZ && l.mode != MODE_MAPPED))
+            return -1;
+    }
     const int RS = l.order / 2;
     if (l.mp && (a.mp_beg != a.j_beg || a.mp_end != a.j_end)) return -1;
     int in_b, out_b;
@@ -626,42 +649,53 @@ int plan_sweep5(TileArgs& a, Launch& l, const void* src)
 
 namespace {
 typedef void (*Sweep5Fn)(const TileArgs);
-template <int ORDER, typename TS, int LAY, bool COPY, bool PAIR, bool MP>
+template <int ORDER, int MODE, typename TS, int LAY, bool COPY, bool PAIR, bool MP>
 Sweep5Fn kernel_of()
 {
     if constexpr (ORDER == 7)
-        return sweep7_kernel<TS, LAY, COPY, PAIR, MP>;
+        return sweep7_kernel<MODE, TS, LAY, COPY, PAIR, MP>;
     else
         return sweep5_kernel<TS, LAY, COPY, PAIR, MP>;
 }
-template <int ORDER, typename TS, bool MP>
+template <int ORDER, int MODE, typename TS, bool MP>
 Sweep5Fn pick5(int lay, bool pair, bool copy)
 {
-    return lay == 0 ? (copy ? kernel_of<ORDER, TS, 0, true, false, MP>() : kernel_of<ORDER, TS, 0, false, false, MP>())
-           : pair   ? (copy ? kernel_of<ORDER, TS, 1, true, true, MP>() : kernel_of<ORDER, TS, 1, false, true, MP>())
-                    : (copy ? kernel_of<ORDER, TS, 1, true, false, MP>() : kernel_of<ORDER, TS, 1, false, false, MP>());
-}
-template <int ORDER, typename TS>
-Sweep5Fn pick5(int lay, bool pair, bool copy, bool mp)
-{
-    return mp ? pick5<ORDER, TS, true>(lay, pair, copy) : pick5<ORDER, TS, false>(lay, pair, copy);
+    return lay == 0 ? (copy ? kernel_of<ORDER, MODE, TS, 0, true, false, MP>()
+                            : kernel_of<ORDER, MODE, TS, 0, false, false, MP>())
+           : pair   ? (copy ? kernel_of<ORDER, MODE, TS, 1, true, true, MP>()
+                            : kernel_of<ORDER, MODE, TS, 1, false, true, MP>())
+                    : (copy ? kernel_of<ORDER, MODE, TS, 1, true, false, MP>()
+                            : kernel_of<ORDER, MODE, TS, 1, false, false, MP>());
 }
 template <int ORDER>
 Sweep5Fn pick5(int dtype, int lay, bool pair, bool copy, bool mp)
 {
-    return dtype == DT_F64   ? pick5<ORDER, double>(lay, pair, copy, mp)
-           : dtype == DT_F32 ? pick5<ORDER, float>(lay, pair, copy, mp)
-                             : pick5<ORDER, bf16_t>(lay, pair, copy, mp);
+    auto by_type = [&](auto ts) {
+        using TS = decltype(ts);
+        return mp ? pick5<ORDER, MODE_JS, TS, true>(lay, pair, copy) : pick5<ORDER, MODE_JS, TS, false>(lay, pair, copy);
+    };
+    return dtype == DT_F64 ? by_type(double{}) : dtype == DT_F32 ? by_type(float{}) : by_type(bf16_t{});
+}
+// WENO7's other weight branches (FP64, no fused limiter)
+Sweep5Fn pick7_mode(int mode, int lay, bool pair, bool copy)
+{
+    switch (mode) {
+    case MODE_TENO: return pick5<7, MODE_TENO, double, false>(lay, pair, copy);
+    case MODE_WENOZ: return pick5<7, MODE_WENOZ, double, false>(lay, pair, copy);
+    default: return pick5<7, MODE_MAPPED, double, false>(lay, pair, copy);
+    }
 }
 }  // namespace
 
-hipError_t launch_sweep5(TileArgs a, int order, int lay, int dtype, int mp, hipStream_t st)
+hipError_t launch_sweep5(TileArgs a, int order, int mode, int lay, int dtype, int mp, hipStream_t st)
 {
     if (a.threads != kThreads5 || 2 * a.jn > kConsumers || a.ldj > 0xffff) return hipErrorInvalidConfiguration;
+    if (mode != MODE_JS && (order != 7 || dtype != DT_F64 || mp)) return hipErrorInvalidConfiguration;
     const size_t lds = 2 * static_cast<size_t>(kNP) * a.ldj * sizeof(double);
     const bool copy = a.copy_halo != 0, pair = a.pair_b != 0;
-    const Sweep5Fn kern = order == 7 ? pick5<7>(dtype, lay, pair, copy, mp != 0)
-                                     : pick5<5>(dtype, lay, pair, copy, mp != 0);
+    const Sweep5Fn kern = mode != MODE_JS ? pick7_mode(mode, lay, pair, copy)
+                          : order == 7    ? pick5<7>(dtype, lay, pair, copy, mp != 0)
+                                          : pick5<5>(dtype, lay, pair, copy, mp != 0);
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
