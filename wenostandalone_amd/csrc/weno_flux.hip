@@ -22,6 +22,7 @@
 // faces that already exist (faces_flux_kernel).
 #include "weno_device.h"
 #include "weno_cells.h"
+#include "weno_tiles.h"
 #include "weno_capi_internal.h"
 
 #include <algorithm>
@@ -61,43 +62,12 @@ __device__ __forceinline__ void cell_faces(const double* v, const Cf<ORDER, doub
         weno7<MODE, double>(v, f, B, eps, ct, oL, oR);
 }
 
-// Tile t of the fused launch: variable iv, plane pb, pencils pa0 ..
-// pa0 + cnt - 1 (tiles never cross a plane: tiles_per_plane per plane).
-__device__ __forceinline__ void plane_tile(const TileArgs& a, int t, int& iv, int& pa0, int& pb, int& cnt)
-{
-    iv = t / a.tiles_per_var;
-    const int tv = t - iv * a.tiles_per_var;
-    pb = tv / a.chunks;
-    pa0 = (tv - pb * a.chunks) * a.np;
-    cnt = min(a.np, a.na - pa0);
-}
-
 // Interface fluxes from neighbouring lanes.  Lane i of wave w holds cell
 // j = 3 + 62 w + i: lanes 1..62 own the rhs of their cells; lanes 0 and 63
 // only supply the faces of the cells on either side of the wave's range, so
 // both interfaces of every owned cell are formed inside one wave with two
 // shuffles and no LDS round trip.
 constexpr int kCellsPerWave = 62;
-
-template <typename T>
-using GlobalIn = __attribute__((address_space(1))) const T;
-
-// Lane i receives lane i+1's value (DPP wave_shl:1) / lane i-1's value
-// (DPP wave_shr:1); the lanes at the wave's ends receive 0.
-__device__ __forceinline__ double from_next_lane(double x)
-{
-    const long long b = __double_as_longlong(x);
-    const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b), 0x130, 0xf, 0xf, true);
-    const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), 0x130, 0xf, 0xf, true);
-    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
-}
-__device__ __forceinline__ double from_prev_lane(double x)
-{
-    const long long b = __double_as_longlong(x);
-    const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b), 0x138, 0xf, 0xf, true);
-    const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), 0x138, 0xf, 0xf, true);
-    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
-}
 
 // Every lane forms F(j+1/2) = F(vL(j), vR(j+1)) with its right neighbour's
 // vR, and takes F(j-1/2) from its left neighbour: one flux evaluation and
@@ -174,72 +144,13 @@ fused_flux_kernel(const FluxArgs fx)
     if (t_first >= t_end) return;
 
     if (producer) {
-        const unsigned long long src = reinterpret_cast<unsigned long long>(a.src);
         double r[kFluxSlots];
-        // Contiguous rows: the tile's rows are np consecutive rows of one
-        // plane, s_a elements apart; the producer loads the span from the
-        // first row's J = lo to the last row's J = lo + nJ - 1 (gaps
-        // included), 64 consecutive elements per instruction, and element e
-        // of the span goes to LDS row e / s_a, column e % s_a.
-        // Strided rows: the tile's np pencils are consecutive in memory;
-        // lane l holds pencil l % np of rows J = l / np + (64 / np) s.
-        const int G = 64 / np;
-        const int sa = static_cast<int>(a.s_a);
-        const float inv_sa = 1.0f / static_cast<float>(sa);
-        // Per-slot offsets do not change from tile to tile; the opaque copy of
-        // the lane index keeps the compiler from holding all of them in
-        // registers across the tile loop.
-        auto issue = [&](int t) {
-            int l = tid & 63;
-            asm volatile("" : "+v"(l));
-            const int sq = l % np, J0 = l / np;
-            int iv, pa0, pb, cnt;
-            plane_tile(a, t, iv, pa0, pb, cnt);
-            const unsigned long long bm = src + 8ull * (a.src_off + pa0 * a.s_a + pb * a.s_b + iv * a.s_i +
-                                                        static_cast<long long>(fx.lo) * a.s_j);
-            if (rows) {
-                const int span = (cnt - 1) * sa + nJ;
-#pragma unroll
-                for (int s = 0; s < kFluxSlots; ++s)
-                    if (l + 64 * s < span)
-                        r[s] = *reinterpret_cast<GlobalIn<double>*>(bm + 8u * static_cast<unsigned>(l + 64 * s));
-            } else {
-                const unsigned off = static_cast<unsigned>(8 * (sq + J0 * a.s_j));
-                const unsigned long long stepb = 8ull * G * a.s_j;
-#pragma unroll
-                for (int s = 0; s < kFluxSlots; ++s)
-                    if (sq < cnt && J0 + G * s < nJ)
-                        r[s] = *reinterpret_cast<GlobalIn<double>*>(bm + stepb * s + off);
-            }
-            return cnt;
-        };
-        auto commit = [&](double* b, int cnt) {
-            int l = tid & 63;
-            asm volatile("" : "+v"(l));
-            const int sq = l % np, J0 = l / np;
-            if (rows) {
-                const int span = (cnt - 1) * sa + nJ;
-#pragma unroll
-                for (int s = 0; s < kFluxSlots; ++s) {
-                    // element e of the span -> row q = e / s_a, exact in FP32
-                    // for e < 2^11 (the quotient is never within 0.5/s_a of
-                    // an integer it does not equal)
-                    const int e = l + 64 * s;
-                    const int q = static_cast<int>((static_cast<float>(e) + 0.5f) * inv_sa), J = e - q * sa;
-                    if (e < span && J < nJ) b[q * ld + J] = r[s];
-                }
-            } else {
-#pragma unroll
-                for (int s = 0; s < kFluxSlots; ++s)
-                    if (sq < cnt && J0 + G * s < nJ) b[sq * ld + J0 + G * s] = r[s];
-            }
-        };
-        commit(buf0, issue(t_first));
+        tile_commit(a, nJ, ld, tile_issue(a, fx.lo, nJ, t_first, r), r, buf0);
         __syncthreads();
         int k = 0;
         for (int t = t_first; t < t_end; t += tstep, ++k) {
             const int tn = t + tstep;
-            if (tn < t_end) commit((k & 1) ? buf0 : buf1, issue(tn));
+            if (tn < t_end) tile_commit(a, nJ, ld, tile_issue(a, fx.lo, nJ, tn, r), r, (k & 1) ? buf0 : buf1);
             __syncthreads();
         }
         return;
@@ -386,22 +297,11 @@ int cu_count()
 // waves of 62 cells covering the rows' rhs range 4 .. ns-5
 int waves_per_row(int ns) { return (ns - 8 + kCellsPerWave - 1) / kCellsPerWave; }
 
-// Pencils per tile (a power of two) for a row of nJ inputs: the producer
-// holds one tile in kFluxSlots registers per lane.
-int fused_np(int nJ, long long s_a, bool rows)
-{
-    for (int np = kFluxMaxNP; np >= 1; np >>= 1) {
-        const long long slots = rows ? ((np - 1) * s_a + nJ + 63) / 64 : (nJ + 64 / np - 1) / (64 / np);
-        if (slots <= kFluxSlots) return np;
-    }
-    return 0;
-}
-
 hipError_t launch_fused(int order, int mode, bool mp, FluxArgs fx, hipStream_t st)
 {
     TileArgs& a = fx.a;
     const FluxFn kern = pick_fused(order, mode, mp);
-    a.np = fused_np(fx.nJ, a.s_a, a.s_j == 1);
+    a.np = tile_pencils(fx.nJ, a.s_a, a.s_j == 1, kFluxSlots, kFluxMaxNP);
     if (a.np < 1) return hipErrorInvalidConfiguration;
     // rows laid end to end: as many as kFluxConsumers waves of 62 cells cover
     const int nf = a.out_len - 6;
