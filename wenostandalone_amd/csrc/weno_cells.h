@@ -11,7 +11,7 @@
 // with four stencils, three differences per candidate and six terms per beta
 // (the 15 products D_i^2, D_i D_(i-1), D_i D_(i-2) shared by all four).
 // cell5/cell7 report when the shared product leaves the normal range; the
-// caller then redoes the cell with cell5_slow/cell7_slow (per-stencil IEEE
+// caller then redoes the cell with cell5_slow_inl/cell7_slow_inl (per-stencil IEEE
 // divisions, which also carry NaN and infinity through as the reference
 // does).
 #pragma once
@@ -82,9 +82,10 @@ __device__ __forceinline__ bool cell5(double v0, double v1, double v2, double v3
     return (static_cast<int>(pos_normal(T)) & static_cast<int>(pos_normal(r))) != 0;
 }
 
-// Per-stencil IEEE form (the reference's alpha_r = d_r / beta_r^2).  The
-// inline form is for kernels that take it per cell; cell5_slow keeps it out
-// of line for kernels that redo whole tiles.
+// Per-stencil IEEE form (the reference's alpha_r = d_r / beta_r^2).  Always
+// inlined: an out-of-line version takes the coefficient struct's address,
+// and the compiler then keeps the caller's coefficient registers in scratch
+// (written to memory at every launch).
 __device__ __forceinline__ void cell5_slow_inl(double v0, double v1, double v2, double v3, double v4, const Coef5& k,
                                         double eps, double& oL, double& oR)
 {
@@ -101,12 +102,6 @@ __device__ __forceinline__ void cell5_slow_inl(double v0, double v1, double v2, 
     const double SR = k.dR[0] * y0 + k.dR[1] * y1 + k.dR[2] * y2;
     oL = v2 + NL / SL;
     oR = v2 + NR / SR;
-}
-
-__device__ __noinline__ void cell5_slow(double v0, double v1, double v2, double v3, double v4, const Coef5& k,
-                                        double eps, double& oL, double& oR)
-{
-    cell5_slow_inl(v0, v1, v2, v3, v4, k, eps, oL, oR);
 }
 
 // WENO7: stencil q (0..3) of the reference's pairing (:1457-1473) uses the
@@ -207,11 +202,6 @@ __device__ __forceinline__ void cell7_slow_inl(const double (&v)[7], const Coef7
     }
     oL = v[3] + NL / SL;
     oR = v[3] + NR / SR;
-}
-
-__device__ __noinline__ void cell7_slow(const double (&v)[7], const Coef7& k, double eps, double& oL, double& oR)
-{
-    cell7_slow_inl(v, k, eps, oL, oR);
 }
 
 }  // namespace

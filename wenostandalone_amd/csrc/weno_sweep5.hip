@@ -515,15 +515,19 @@ __device__ __forceinline__ void sweep_body(const TileArgs& a)
                     const int q = 2 * qq + gq;
                     const double* row = cur + 2 * qq * ldj + colc;
                     double oL, oR;
+                    // inline: an out-of-line call takes the coefficient
+                    // struct's address, and the compiler then keeps the
+                    // lane's 27 (56) coefficients in scratch: 240 bytes per
+                    // lane written at the start of every launch (63 MB of
+                    // HBM writes per WENO5 sweep at 200^3 x 8), and for
+                    // WENO7 reloaded every tile
                     if constexpr (ORDER == 7) {
-                        // inline: an out-of-line call would keep the 56
-                        // coefficients in scratch and reload them every tile
                         double w[7];
 #pragma unroll
                         for (int u = 0; u < 7; ++u) w[u] = row[u - 3];
                         if constexpr (MODE == MODE_JS) cell7_slow_inl(w, k5, eps, oL, oR);
                     } else {
-                        cell5_slow(row[-2], row[-1], row[0], row[1], row[2], k5, eps, oL, oR);
+                        cell5_slow_inl(row[-2], row[-1], row[0], row[1], row[2], k5, eps, oL, oR);
                     }
                     if constexpr (MP) {
                         const double w[5] = {row[-2], row[-1], row[0], row[1], row[2]};
