@@ -9,13 +9,14 @@
 //
 // One workgroup = 7 consumer waves + 1 producer wave, persistent over a
 // sequence of tiles.  A tile is 16 pencils x one j-chunk of at most 224
-// outputs; tiles flow through two LDS buffers.  Tiles of contiguous rows (x
-// sweeps, reshaped sources), which share no input lines, go round-robin over
-// the whole grid, so that all workgroups stream through one narrow window of
-// the input and output arrays (3 % faster than one range per XCD).  Tiles of
-// y and z sweeps share input lines with their neighbours (y: 16 x-values
-// start 24 bytes into a 128-byte line; z: paired 8 x 2 tiles) and keep one
-// contiguous range per XCD, so the shared lines meet in one L2.
+// outputs; tiles flow through two LDS buffers.  Tiles of x and y sweeps (and
+// reshaped sources) go round-robin over the whole grid, so that all
+// workgroups stream through one narrow window of the input and output arrays
+// (x: 3 % faster than one range per XCD; y: 408 -> 398 us per 200^3 x 8
+// sweep, although the input lines that neighbouring y tiles share are then
+// fetched into two L2s: the second fetch hits the Infinity Cache).  z sweeps
+// (paired 8 x 2 tiles, whose output rows lie 359 KB apart per x-value) keep
+// one contiguous range per XCD: round-robin made them 3 % slower.
 //
 //   producer wave : vector loads of tile k+1 from HBM into registers, then
 //                   into LDS as [pencil][J] (the y/z transposes happen here);
@@ -251,7 +252,7 @@ __device__ __forceinline__ void sweep_body(const TileArgs& a)
     const int xcd = bin & 7, rank = bin >> 3, nbx = bpc >> 3;
     const int per = ntile >> 3, rem = ntile & 7;
     // workgroup bin's tiles: t_first, t_first + tstep, ... < t_end
-    constexpr bool kGlobal = LAY == 0;
+    constexpr bool kGlobal = !PAIR;
     const int t_first = kGlobal ? bin : xcd * per + min(xcd, rem) + rank;
     const int t_end = kGlobal ? ntile : (xcd + 1) * per + min(xcd + 1, rem);
     const int tstep = kGlobal ? bpc : nbx;
