@@ -208,9 +208,13 @@ __device__ __forceinline__ long long out_base(const TileArgs& a, int iv, int pa,
 //        of a tile are consecutive along x, loaded as 128-byte row segments
 //        (one pencil per lane, 4 rows per wave instruction) and transposed
 //        into LDS.
-// WENO7 weight branches other than JS: the raw candidate and smoothness
-// coefficients and linear weights of the lane, in registers, for the
-// general per-family arithmetic (weno7 in weno_device.h).
+// Weight branches other than JS (TENO, WENO-Z, mapped): the raw candidate
+// and smoothness coefficients and linear weights of the lane, in registers,
+// for the general per-family arithmetic (weno5 / weno7 in weno_device.h).
+struct Gen5 {
+    Cf<5, double> f;
+    BetaRegs<5> b;
+};
 struct Gen7 {
     Cf<7, double> f;
     BetaRegs<7> b;
@@ -415,13 +419,15 @@ __device__ __forceinline__ void sweep_body(const TileArgs& a)
     // halo lanes run the stencil on a clamped window and keep their value
     const int colc = ORDER == 5 ? min(max(col, 2), nrow - 3) : min(max(col, RS), nrow - 1 - RS);
     const int hpos = j < a.j_beg ? j - (a.j_beg - 3) : j - (a.j_end + 1) + 3;
-    std::conditional_t<ORDER == 7, std::conditional_t<MODE == MODE_JS, Coef7, Gen7>, Coef5> k5;
+    std::conditional_t<ORDER == 7, std::conditional_t<MODE == MODE_JS, Coef7, Gen7>,
+                       std::conditional_t<MODE == MODE_JS, Coef5, Gen5>>
+        k5;
     {
         int jc = j - 3;
         jc = jc < 0 ? 0 : (jc > a.c.nc - 1 ? a.c.nc - 1 : jc);
-        if constexpr (ORDER == 5) {
+        if constexpr (ORDER == 5 && MODE == MODE_JS) {
             load_coef5(a.c, jc, k5);
-        } else if constexpr (MODE == MODE_JS) {
+        } else if constexpr (ORDER == 7 && MODE == MODE_JS) {
             load_coef7(a.c, jc, k5);
         } else {
             load_cf(a.c, jc, k5.f);
@@ -451,7 +457,14 @@ __device__ __forceinline__ void sweep_body(const TileArgs& a)
                     const double* row = cur + 2 * qq * ldj + colc;
                     const double v0 = row[-2], v1 = row[-1], v2 = row[0], v3 = row[1], v4 = row[2];
                     double oL, oR;
-                    allok &= cell5<NEWTON>(v0, v1, v2, v3, v4, k5, eps, oL, oR);
+                    if constexpr (MODE == MODE_JS) {
+                        allok &= cell5<NEWTON>(v0, v1, v2, v3, v4, k5, eps, oL, oR);
+                    } else {
+                        // the general kernel's per-family arithmetic, which
+                        // carries every input range itself
+                        const double w[5] = {v0, v1, v2, v3, v4};
+                        weno5<MODE, double>(w, k5.f, k5.b, eps, a.teno_ct, oL, oR);
+                    }
                     if constexpr (MP) {
                         // every lane limits (halo lanes replace their value below)
                         const double w[5] = {v0, v1, v2, v3, v4};
@@ -527,7 +540,7 @@ __device__ __forceinline__ void sweep_body(const TileArgs& a)
 #pragma unroll
                         for (int u = 0; u < 7; ++u) w[u] = row[u - 3];
                         if constexpr (MODE == MODE_JS) cell7_slow_inl(w, k5, eps, oL, oR);
-                    } else {
+                    } else if constexpr (MODE == MODE_JS) {
                         cell5_slow_inl(row[-2], row[-1], row[0], row[1], row[2], k5, eps, oL, oR);
                     }
                     if constexpr (MP) {
@@ -552,6 +565,15 @@ sweep5_kernel(const TileArgs a)
     sweep_body<5, MODE_JS, TS, LAY, COPY, PAIR, MP>(a);
 }
 
+// WENO5's other weight branches (TENO, WENO-Z, mapped) for FP64 state
+// without the fused limiter, at the WENO5-JS kernel's occupancy.
+template <int MODE, int LAY, bool COPY, bool PAIR>
+__global__ void __attribute__((amdgpu_flat_work_group_size(64, kThreads5), amdgpu_waves_per_eu(4)))
+sweep5_branch_kernel(const TileArgs a)
+{
+    sweep_body<5, MODE, double, LAY, COPY, PAIR, false>(a);
+}
+
 // The WENO7 kernel: the 56 coefficients of a lane take 112 VGPRs, so it
 // runs at 2 waves per SIMD (one workgroup per CU).  JS for every storage
 // type, with or without the fused limiter; the other weight branches for
@@ -563,9 +585,9 @@ sweep7_kernel(const TileArgs a)
     sweep_body<7, MODE, TS, LAY, COPY, PAIR, MP>(a);
 }
 
-// Host side: decides whether a WENO5-JS or WENO7-JS launch runs on these
-// kernels and sets
-// its tile shape.  Returns the layout (0 or 1), or -1 when the general sweep
+// Host side: decides whether a WENO5 or WENO7 launch (JS for every storage
+// type; TENO, WENO-Z and mapped for FP64 state without the fused limiter)
+// runs on these kernels and sets its tile shape.  Returns the layout (0 or 1), or -1 when the general sweep
 // kernel must be used (other orders/modes, a fused limiter range other than
 // the reconstructed range, one-sided output, rows shorter than 96 outputs,
 // odd row lengths or misaligned arrays, arrays too large for 32-bit
@@ -574,8 +596,8 @@ int plan_sweep5(TileArgs& a, Launch& l, const void* src)
 {
     if (l.op != OP_WENO || (l.order != 5 && l.order != 7) || a.sides != 3) return -1;
     if (l.mode != MODE_JS) {
-        // WENO7's other weight branches: FP64 state, no fused limiter
-        if (l.order != 7 || l.dtype != DT_F64 || l.mp ||
+        // the other weight branches: FP64 state, no fused limiter
+        if (l.dtype != DT_F64 || l.mp ||
             (l.mode != MODE_TENO && l.mode != MODE_WENOZ && l.mode != MODE_MAPPED))
             return -1;
     }
@@ -659,6 +681,8 @@ Sweep5Fn kernel_of()
 {
     if constexpr (ORDER == 7)
         return sweep7_kernel<MODE, TS, LAY, COPY, PAIR, MP>;
+    else if constexpr (MODE != MODE_JS)
+        return sweep5_branch_kernel<MODE, LAY, COPY, PAIR>;
     else
         return sweep5_kernel<TS, LAY, COPY, PAIR, MP>;
 }
@@ -681,13 +705,14 @@ Sweep5Fn pick5(int dtype, int lay, bool pair, bool copy, bool mp)
     };
     return dtype == DT_F64 ? by_type(double{}) : dtype == DT_F32 ? by_type(float{}) : by_type(bf16_t{});
 }
-// WENO7's other weight branches (FP64, no fused limiter)
-Sweep5Fn pick7_mode(int mode, int lay, bool pair, bool copy)
+// the other weight branches (FP64, no fused limiter)
+template <int ORDER>
+Sweep5Fn pick_mode(int mode, int lay, bool pair, bool copy)
 {
     switch (mode) {
-    case MODE_TENO: return pick5<7, MODE_TENO, double, false>(lay, pair, copy);
-    case MODE_WENOZ: return pick5<7, MODE_WENOZ, double, false>(lay, pair, copy);
-    default: return pick5<7, MODE_MAPPED, double, false>(lay, pair, copy);
+    case MODE_TENO: return pick5<ORDER, MODE_TENO, double, false>(lay, pair, copy);
+    case MODE_WENOZ: return pick5<ORDER, MODE_WENOZ, double, false>(lay, pair, copy);
+    default: return pick5<ORDER, MODE_MAPPED, double, false>(lay, pair, copy);
     }
 }
 }  // namespace
@@ -695,10 +720,11 @@ Sweep5Fn pick7_mode(int mode, int lay, bool pair, bool copy)
 hipError_t launch_sweep5(TileArgs a, int order, int mode, int lay, int dtype, int mp, hipStream_t st)
 {
     if (a.threads != kThreads5 || 2 * a.jn > kConsumers || a.ldj > 0xffff) return hipErrorInvalidConfiguration;
-    if (mode != MODE_JS && (order != 7 || dtype != DT_F64 || mp)) return hipErrorInvalidConfiguration;
+    if (mode != MODE_JS && (dtype != DT_F64 || mp)) return hipErrorInvalidConfiguration;
     const size_t lds = 2 * static_cast<size_t>(kNP) * a.ldj * sizeof(double);
     const bool copy = a.copy_halo != 0, pair = a.pair_b != 0;
-    const Sweep5Fn kern = mode != MODE_JS ? pick7_mode(mode, lay, pair, copy)
+    const Sweep5Fn kern = mode != MODE_JS ? (order == 7 ? pick_mode<7>(mode, lay, pair, copy)
+                                                        : pick_mode<5>(mode, lay, pair, copy))
                           : order == 7    ? pick5<7>(dtype, lay, pair, copy, mp != 0)
                                           : pick5<5>(dtype, lay, pair, copy, mp != 0);
     if (lds > 64 * 1024) {
