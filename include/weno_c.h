@@ -255,7 +255,7 @@ int weno_stream_copy(void* dst, const void* src, size_t bytes);
  * outputs per tile along the sweep, threads per workgroup (0 = automatic
  * for each) and the kernel variant (-1 = automatic; otherwise bits: 1 load
  * the next tile during compute, 2 WENO5 coefficients in registers, 4 one
- * tile per workgroup, 8 keep WENO5-JS on the general kernel instead of the
+ * tile per workgroup, 8 keep the launch on the general kernel instead of the
  * producer/consumer kernel, whose tile shape is fixed; the general kernel's
  * FP32/BF16-state form does its stencil arithmetic in FP32).  Used by the
  * autotuner; no reference counterpart. */

@@ -416,7 +416,7 @@ int build_reconstruct(const weno_grid* g, int order, int dir, int src_kind, int 
     const int variant = tuned_prefetch(order, dtype, dir);
     plan_tiles(a, l, tuned_np(order, dtype, dir), tuned_chunk(order, dtype, dir), tuned_threads(order, dtype, dir),
                variant);
-    // variant bit 8 keeps WENO5-JS on the general kernel
+    // variant bit 8 keeps the launch on the general kernel
     if (variant < 0 || !(variant & 8)) l.fast5 = plan_sweep5(a, l, s) + 1;
     return WENO_OK;
 }
