@@ -12,9 +12,9 @@
 // outputs; tiles flow through two LDS buffers.  Tiles of x and y sweeps (and
 // reshaped sources) go round-robin over the whole grid, so that all
 // workgroups stream through one narrow window of the input and output arrays
-// (x: 3 % faster than one range per XCD; y: 408 -> 398 us per 200^3 x 8
-// sweep, although the input lines that neighbouring y tiles share are then
-// fetched into two L2s: the second fetch hits the Infinity Cache).  z sweeps
+// (x: 3 % faster than one range per XCD; y: 408 -> 393 us per 200^3 x 8
+// sweep, with groups of 16 neighbouring y tiles, which share input lines, on
+// the workgroups of one XCD).  z sweeps
 // (paired 8 x 2 tiles, whose output rows lie 359 KB apart per x-value) keep
 // one contiguous range per XCD: round-robin made them 3 % slower.
 //
@@ -257,7 +257,13 @@ __device__ __forceinline__ void sweep_body(const TileArgs& a)
     const int per = ntile >> 3, rem = ntile & 7;
     // workgroup bin's tiles: t_first, t_first + tstep, ... < t_end
     constexpr bool kGlobal = !PAIR;
-    const int t_first = kGlobal ? bin : xcd * per + min(xcd, rem) + rank;
+    // y: each round of bpc tiles gives groups of 16 neighbouring tiles (which
+    // share input lines) to the workgroups of one XCD, so a shared line is
+    // fetched into one L2 (groups of 1 / 16: L2 fetch 1370 / 804 MB per
+    // 200^3 x 8 sweep, 396 / 393 us)
+    constexpr int G = (LAY == 1 && !PAIR) ? 16 : 1;
+    const int gbin = (G > 1 && nbx % G == 0) ? G * (xcd + 8 * (rank / G)) + rank % G : bin;
+    const int t_first = kGlobal ? gbin : xcd * per + min(xcd, rem) + rank;
     const int t_end = kGlobal ? ntile : (xcd + 1) * per + min(xcd + 1, rem);
     const int tstep = kGlobal ? bpc : nbx;
     if (t_first >= t_end) return;
