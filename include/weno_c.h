@@ -215,7 +215,7 @@ int weno_batched_reconstruct(const weno_block* blocks, int nblocks, int dir, int
  * graph capture, tight serving loops, harnesses that forbid hipMalloc).
  * The work-item counter and the block descriptors live in a library-owned
  * static device array (a 256-byte counter slot and WENO_NOALLOC_MAX_BLOCKS
- * descriptors of 288 bytes: 73,984 bytes per GPU, held until the process
+ * descriptors of 280 bytes: 71,936 bytes per GPU, held until the process
  * ends) instead of memory the library allocates; more than
  * WENO_NOALLOC_MAX_BLOCKS blocks is WENO_E_INVALID.
  * Every call on one GPU shares that array: calls that overlap on one GPU

@@ -75,16 +75,18 @@ def kernels(tmp_path_factory):
 def test_sweep_kernels_hold_no_coefficient_frame(kernels):
     sweeps = {n: v for n, v in kernels.items()
               if "13sweep5_kernel" in n or "13sweep7_kernel" in n or "20sweep5_branch_kernel" in n}
-    # 2 orders x storage types x layouts x copy/core x pair x limiter, and the weight branches
-    assert len(sweeps) >= 66, len(sweeps)
+    # JS: 2 orders x 3 storage types x 2 layouts x copy/core x limiter (48);
+    # TENO / WENO-Z / mapped: 3 branches x 2 orders x 2 layouts x copy/core (24)
+    assert len(sweeps) >= 72, len(sweeps)
     heavy = {n: s for n, (s, _) in sweeps.items() if s > 32}
     assert not heavy, heavy
 
 
 def test_headline_sweep_kernels_have_no_scratch(kernels):
-    # WENO5-JS FP64 core-only x and y sweeps (the bench workload's x and y launches)
-    for name in ("_ZN2wm13sweep5_kernelIdLi0ELb0ELb0ELb0EEEvNS_8TileArgsE",
-                 "_ZN2wm13sweep5_kernelIdLi1ELb0ELb0ELb0EEEvNS_8TileArgsE"):
+    # WENO5-JS FP64 core-only sweeps along the contiguous and a strided axis (the
+    # bench workload's x launch, and its y and z launches)
+    for name in ("_ZN2wm13sweep5_kernelIdLi0ELb0ELb0EEEvNS_8TileArgsE",
+                 "_ZN2wm13sweep5_kernelIdLi1ELb0ELb0EEEvNS_8TileArgsE"):
         assert name in kernels, name
         assert kernels[name][0] == 0, (name, kernels[name])
         # 4 waves per SIMD: two 512-thread workgroups per CU

@@ -3,8 +3,7 @@
 sweep7_kernel for WENO7, both planned by plan_sweep5 for rows of at least 96
 outputs), against the oracle:
 
-  * extents that leave part-filled tiles in the x, y and z sweeps (z tiles
-    pair adjacent y-rows);
+  * extents that leave part-filled tiles in the x, y and z sweeps;
   * rows of two j-chunks (312 cells along each axis in turn);
   * field and reshaped sources; core-only launches keep arbitrary values in
     the halo cells of vL/vR bit for bit, fused-copy launches write the
@@ -97,8 +96,8 @@ def _check(branch, order, m, n, p, v, dirs, seed):
 @pytest.mark.parametrize("order", [5, 7])
 @pytest.mark.parametrize("branch", list(BRANCH_KW))
 def test_branch_sweep_partial_tiles(branch, order):
-    # nx, ny, nz = 150, 146, 138 (even, so every sweep takes the kernel): the x
-    # and y sweeps' 16-pencil tiles and the z sweep's 8-x-value tiles end part-filled
+    # nx, ny, nz = 150, 146, 138 (even, so every sweep takes the kernel): the
+    # 16-pencil tiles end part-filled and span two transverse rows
     _check(branch, order, 137, 133, 125, 2, (0, 1, 2), seed=700 + order)
 
 

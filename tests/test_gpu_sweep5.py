@@ -11,8 +11,9 @@ weno5 followed by its MP):
     bitwise); core-only launches rewrite the halo cells with the values they
     hold, so arbitrary halo contents must come back bit for bit;
   * rows longer than one tile (ns = 312 along each axis in turn: two chunks),
-    extents that leave partial tiles (z sweeps tile 8 x-values x 2 adjacent
-    y-rows, so that the two lane groups write adjacent output rows);
+    extents that leave partial tiles (16 pencils consecutive along the
+    transverse axis of smallest input stride, so with an odd extent there a
+    tile spans two transverse rows);
   * against the general kernel (variant bit 8) on the same inputs;
   * fields scaled by 1e22 and 1e-30 (eps 1e-100): the shared denominator
     (sum_L)(sum_R) over- or underflows, so lanes take the per-stencil IEEE
@@ -150,8 +151,8 @@ def test_core_only_keeps_arbitrary_halo():
 
 
 def test_odd_extents_and_partial_tiles():
-    """nx = 153 and ny = 143: partial 16-pencil tiles (x, y sweeps) and
-    partial 8 x 2 tiles of the z sweep (odd y count)."""
+    """nx = 153, ny = 143, nz = 140: partial 16-pencil tiles in every
+    sweep, and tiles that span two transverse rows."""
     g, og, f = _case(140, 130, 127, 1, seed=512)
     _check_box(g, og, f, seed=21)
 

@@ -34,7 +34,7 @@ constexpr int kNoallocMaxBlocks = WENO_NOALLOC_MAX_BLOCKS;
 constexpr size_t kNoallocCounterBytes = 256;
 constexpr size_t kNoallocScratchBytes = kNoallocCounterBytes + size_t(kNoallocMaxBlocks) * sizeof(BatchDesc);
 static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
-static_assert(sizeof(BatchDesc) == 288, "the header states the scratch size from a 288-byte descriptor");
+static_assert(sizeof(BatchDesc) == 280, "the header states the scratch size from a 280-byte descriptor");
 
 namespace {
 

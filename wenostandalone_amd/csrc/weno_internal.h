@@ -57,7 +57,6 @@ struct TileArgs {
     int bpc;                      // blocks per chunk (set at launch)
     int ldj;                      // LDS row stride (elements)
     unsigned long long src_bytes; // size of the input array (all variables)
-    int pair_b;                   // tiles of 8 a-values x 2 adjacent b-values
     Coefs c;
     double eps, teno_ct;
 };

@@ -9,14 +9,14 @@
 //
 // One workgroup = 7 consumer waves + 1 producer wave, persistent over a
 // sequence of tiles.  A tile is 16 pencils x one j-chunk of at most 224
-// outputs; tiles flow through two LDS buffers.  Tiles of x and y sweeps (and
-// reshaped sources) go round-robin over the whole grid, so that all
-// workgroups stream through one narrow window of the input and output arrays
-// (x: 3 % faster than one range per XCD; y: 408 -> 393 us per 200^3 x 8
-// sweep, with groups of 16 neighbouring y tiles, which share input lines, on
-// the workgroups of one XCD).  z sweeps
-// (paired 8 x 2 tiles, whose output rows lie 359 KB apart per x-value) keep
-// one contiguous range per XCD: round-robin made them 3 % slower.
+// outputs; tiles flow through two LDS buffers.  Tiles go round-robin over
+// the whole grid, so that all workgroups stream through one narrow window of
+// the input and output arrays; in y and z sweeps each round hands groups of
+// 32 neighbouring tiles, which share input lines, to the workgroups of one
+// XCD.  Per 200^3 x 8 sweep: x 3 % faster than one tile range per XCD; y
+// 408 us with one range per XCD, 396 us round-robin, 393 us in groups;
+// z 461 us with tiles of 8 x-values x 2 y-rows (adjacent output rows) and
+// one range per XCD, 448 us with 16 x-values per tile in groups.
 //
 //   producer wave : vector loads of tile k+1 from HBM into registers, then
 //                   into LDS as [pencil][J] (the y/z transposes happen here);
@@ -167,24 +167,14 @@ __device__ __forceinline__ void tile_origin(const TileArgs& a, int t, int& iv, i
     tv = t - iv * a.tiles_per_var;
 }
 
-// Pencil q of tile tv -> transverse coordinates (a, b).  Pencils past the
-// end repeat the last valid one (same data, same results, same addresses).
-// Default tiles are 16 consecutive pencils in (a + na*b) order; paired tiles
-// (z sweeps) are 8 a-values x 2 adjacent b-values with q = 2*(a offset) + (b
-// offset), so pencils 2i and 2i+1 are adjacent output rows.
-template <bool PAIR>
+// Pencil q of tile tv -> transverse coordinates (a, b): tiles are 16
+// consecutive pencils in (a + na*b) order.  Pencils past the end repeat the
+// last valid one (same data, same results, same addresses).
 __device__ __forceinline__ void tile_pencil(const TileArgs& a, int tv, int q, int& pa, int& pb)
 {
-    if constexpr (PAIR) {
-        const int ntx = (a.na + 7) >> 3;
-        const int ty = tv / ntx, tx = tv - ty * ntx;
-        pa = min(8 * tx + (q >> 1), a.na - 1);
-        pb = min(2 * ty + (q & 1), a.nb - 1);
-    } else {
-        const int P = min(tv * kNP + q, a.na * a.nb - 1);
-        pb = P / a.na;
-        pa = P - pb * a.na;
-    }
+    const int P = min(tv * kNP + q, a.na * a.nb - 1);
+    pb = P / a.na;
+    pa = P - pb * a.na;
 }
 
 // Input element offset of pencil (pa, pb) of variable iv at J = lo.
@@ -220,7 +210,7 @@ struct Gen7 {
     BetaRegs<7> b;
 };
 
-template <int ORDER, int MODE, typename TS, int LAY, bool COPY, bool PAIR, bool MP>
+template <int ORDER, int MODE, typename TS, int LAY, bool COPY, bool MP>
 __device__ __forceinline__ void sweep_body(const TileArgs& a)
 {
     constexpr int RS = ORDER / 2;   // stencil radius
@@ -254,19 +244,15 @@ __device__ __forceinline__ void sweep_body(const TileArgs& a)
     const int bin = blockIdx.x - ch * bpc;
     const int ntile = a.tiles_per_var * a.nv;
     const int xcd = bin & 7, rank = bin >> 3, nbx = bpc >> 3;
-    const int per = ntile >> 3, rem = ntile & 7;
-    // workgroup bin's tiles: t_first, t_first + tstep, ... < t_end
-    constexpr bool kGlobal = !PAIR;
-    // y: each round of bpc tiles gives groups of 16 neighbouring tiles (which
-    // share input lines) to the workgroups of one XCD, so a shared line is
-    // fetched into one L2 (groups of 1 / 16: L2 fetch 1370 / 804 MB per
-    // 200^3 x 8 sweep, 396 / 393 us)
-    constexpr int G = (LAY == 1 && !PAIR) ? 16 : 1;
-    const int gbin = (G > 1 && nbx % G == 0) ? G * (xcd + 8 * (rank / G)) + rank % G : bin;
-    const int t_first = kGlobal ? gbin : xcd * per + min(xcd, rem) + rank;
-    const int t_end = kGlobal ? ntile : (xcd + 1) * per + min(xcd + 1, rem);
-    const int tstep = kGlobal ? bpc : nbx;
-    if (t_first >= t_end) return;
+    // workgroup bin's tiles: t_first, t_first + bpc, ... < ntile.  Strided
+    // layouts: each round of bpc tiles gives groups of G neighbouring tiles
+    // (which share input lines: 16 x-values start 24 bytes into a 128-byte
+    // line) to the workgroups of one XCD, so a shared line is fetched into
+    // one L2 (y, groups of 1 / 16 / 32: L2 fetch 1370 / 805 / 775 MB per
+    // 200^3 x 8 sweep)
+    constexpr int G = LAY == 1 ? 32 : 1;
+    const int t_first = (G > 1 && nbx % G == 0) ? G * (xcd + 8 * (rank / G)) + rank % G : bin;
+    if (t_first >= ntile) return;
 
     const int jo0 = ch * a.chunk;
     const int jo1 = min(a.out_len, jo0 + a.chunk);
@@ -308,7 +294,7 @@ __device__ __forceinline__ void sweep_body(const TileArgs& a)
                 const int hj = hpc < 2 ? 2 * hpc : a.out_len - 4 + 2 * (hpc - 2);
                 const bool hin = hj >= jo0 && hj < jo1;
                 int pa, pb;
-                tile_pencil<PAIR>(a, tv, hq, pa, pb);
+                tile_pencil(a, tv, hq, pa, pb);
                 hoff = static_cast<unsigned>((out_base(a, iv, pa, pb) + (hin ? hj : jo0)) * sizeof(TO));
             }
             if constexpr (LAY == 0) {
@@ -323,7 +309,7 @@ __device__ __forceinline__ void sweep_body(const TileArgs& a)
 #pragma unroll
                 for (int h = 0; h < NH; ++h) {
                     int pa, pb;
-                    tile_pencil<PAIR>(a, tv, l / LPR + RPW * h, pa, pb);
+                    tile_pencil(a, tv, l / LPR + RPW * h, pa, pb);
                     const unsigned voff = static_cast<unsigned>(in_base(a, iv, pa, pb, Lb) * sizeof(TS)) +
                                           unsigned(PB) * (l % LPR);
 #pragma unroll
@@ -335,8 +321,8 @@ __device__ __forceinline__ void sweep_body(const TileArgs& a)
                 const long long step = kRPI * a.s_j * static_cast<long long>(sizeof(TS));
                 const int nm = nrow / kRPI;
                 int pa0, pb0, pa, pb;
-                tile_pencil<PAIR>(a, tv, 0, pa0, pb0);
-                tile_pencil<PAIR>(a, tv, q, pa, pb);
+                tile_pencil(a, tv, 0, pa0, pb0);
+                tile_pencil(a, tv, q, pa, pb);
                 const long long b0 = in_base(a, iv, pa0, pb0, Lb);
                 const unsigned lane_off = static_cast<unsigned>(
                     (in_base(a, iv, pa, pb, Lb) - b0 + J0 * a.s_j) * sizeof(TS));
@@ -396,7 +382,7 @@ __device__ __forceinline__ void sweep_body(const TileArgs& a)
             }
             if (l < kNP) {
                 int pa, pb;
-                tile_pencil<PAIR>(a, tv, l, pa, pb);
+                tile_pencil(a, tv, l, pa, pb);
                 s_off[slot][l] = static_cast<unsigned>(out_base(a, iv, pa, pb) * sizeof(TO));
             }
         };
@@ -404,9 +390,9 @@ __device__ __forceinline__ void sweep_body(const TileArgs& a)
         commit(t_first, buf0, 0);
         __syncthreads();
         int k = 0;
-        for (int t = t_first; t < t_end; t += tstep, ++k) {
-            const int tn = t + tstep;
-            if (tn < t_end) {
+        for (int t = t_first; t < ntile; t += bpc, ++k) {
+            const int tn = t + bpc;
+            if (tn < ntile) {
                 tile_in(tn);
                 commit(tn, (k & 1) ? buf0 : buf1, (k + 1) & 1);
             }
@@ -450,7 +436,7 @@ __device__ __forceinline__ void sweep_body(const TileArgs& a)
 
     __syncthreads();
     int k = 0;
-    for (int t = t_first; t < t_end; t += tstep, ++k) {
+    for (int t = t_first; t < ntile; t += bpc, ++k) {
         const int slot = k & 1;
         // group g: pencils g, g+2, g+4, ...
         const double* const cur = ((k & 1) ? buf1 : buf0) + gq * ldj;
@@ -564,31 +550,31 @@ __device__ __forceinline__ void sweep_body(const TileArgs& a)
 }
 
 // The WENO5-JS kernel: 128 VGPRs, 4 waves per SIMD (two workgroups per CU).
-template <typename TS, int LAY, bool COPY, bool PAIR, bool MP>
+template <typename TS, int LAY, bool COPY, bool MP>
 __global__ void __attribute__((amdgpu_flat_work_group_size(64, kThreads5), amdgpu_waves_per_eu(4)))
 sweep5_kernel(const TileArgs a)
 {
-    sweep_body<5, MODE_JS, TS, LAY, COPY, PAIR, MP>(a);
+    sweep_body<5, MODE_JS, TS, LAY, COPY, MP>(a);
 }
 
 // WENO5's other weight branches (TENO, WENO-Z, mapped) for FP64 state
 // without the fused limiter, at the WENO5-JS kernel's occupancy.
-template <int MODE, int LAY, bool COPY, bool PAIR>
+template <int MODE, int LAY, bool COPY>
 __global__ void __attribute__((amdgpu_flat_work_group_size(64, kThreads5), amdgpu_waves_per_eu(4)))
 sweep5_branch_kernel(const TileArgs a)
 {
-    sweep_body<5, MODE, double, LAY, COPY, PAIR, false>(a);
+    sweep_body<5, MODE, double, LAY, COPY, false>(a);
 }
 
 // The WENO7 kernel: the 56 coefficients of a lane take 112 VGPRs, so it
 // runs at 2 waves per SIMD (one workgroup per CU).  JS for every storage
 // type, with or without the fused limiter; the other weight branches for
 // FP64 state without it.
-template <int MODE, typename TS, int LAY, bool COPY, bool PAIR, bool MP>
+template <int MODE, typename TS, int LAY, bool COPY, bool MP>
 __global__ void __attribute__((amdgpu_flat_work_group_size(64, kThreads5), amdgpu_waves_per_eu(2)))
 sweep7_kernel(const TileArgs a)
 {
-    sweep_body<7, MODE, TS, LAY, COPY, PAIR, MP>(a);
+    sweep_body<7, MODE, TS, LAY, COPY, MP>(a);
 }
 
 // Host side: decides whether a WENO5 or WENO7 launch (JS for every storage
@@ -671,10 +657,8 @@ int plan_sweep5(TileArgs& a, Launch& l, const void* src)
         a.npg = kNPG;
         a.ldj = lay == 0 ? kRows0 : (rows | 1);
         a.threads = kThreads5;
-        // z sweeps: output rows are adjacent along b, so tiles pair b-rows
-        a.pair_b = (lay == 1 && !a.a_is_k) ? 1 : 0;
         const long long pen = static_cast<long long>(a.na) * a.nb;
-        a.tiles_per_var = a.pair_b ? ((a.na + 7) / 8) * ((a.nb + 1) / 2) : static_cast<int>((pen + kNP - 1) / kNP);
+        a.tiles_per_var = static_cast<int>((pen + kNP - 1) / kNP);
         return lay;
     }
     return -1;
@@ -682,43 +666,39 @@ int plan_sweep5(TileArgs& a, Launch& l, const void* src)
 
 namespace {
 typedef void (*Sweep5Fn)(const TileArgs);
-template <int ORDER, int MODE, typename TS, int LAY, bool COPY, bool PAIR, bool MP>
+template <int ORDER, int MODE, typename TS, int LAY, bool COPY, bool MP>
 Sweep5Fn kernel_of()
 {
     if constexpr (ORDER == 7)
-        return sweep7_kernel<MODE, TS, LAY, COPY, PAIR, MP>;
+        return sweep7_kernel<MODE, TS, LAY, COPY, MP>;
     else if constexpr (MODE != MODE_JS)
-        return sweep5_branch_kernel<MODE, LAY, COPY, PAIR>;
+        return sweep5_branch_kernel<MODE, LAY, COPY>;
     else
-        return sweep5_kernel<TS, LAY, COPY, PAIR, MP>;
+        return sweep5_kernel<TS, LAY, COPY, MP>;
 }
 template <int ORDER, int MODE, typename TS, bool MP>
-Sweep5Fn pick5(int lay, bool pair, bool copy)
+Sweep5Fn pick5(int lay, bool copy)
 {
-    return lay == 0 ? (copy ? kernel_of<ORDER, MODE, TS, 0, true, false, MP>()
-                            : kernel_of<ORDER, MODE, TS, 0, false, false, MP>())
-           : pair   ? (copy ? kernel_of<ORDER, MODE, TS, 1, true, true, MP>()
-                            : kernel_of<ORDER, MODE, TS, 1, false, true, MP>())
-                    : (copy ? kernel_of<ORDER, MODE, TS, 1, true, false, MP>()
-                            : kernel_of<ORDER, MODE, TS, 1, false, false, MP>());
+    return lay == 0 ? (copy ? kernel_of<ORDER, MODE, TS, 0, true, MP>() : kernel_of<ORDER, MODE, TS, 0, false, MP>())
+                    : (copy ? kernel_of<ORDER, MODE, TS, 1, true, MP>() : kernel_of<ORDER, MODE, TS, 1, false, MP>());
 }
 template <int ORDER>
-Sweep5Fn pick5(int dtype, int lay, bool pair, bool copy, bool mp)
+Sweep5Fn pick5(int dtype, int lay, bool copy, bool mp)
 {
     auto by_type = [&](auto ts) {
         using TS = decltype(ts);
-        return mp ? pick5<ORDER, MODE_JS, TS, true>(lay, pair, copy) : pick5<ORDER, MODE_JS, TS, false>(lay, pair, copy);
+        return mp ? pick5<ORDER, MODE_JS, TS, true>(lay, copy) : pick5<ORDER, MODE_JS, TS, false>(lay, copy);
     };
     return dtype == DT_F64 ? by_type(double{}) : dtype == DT_F32 ? by_type(float{}) : by_type(bf16_t{});
 }
 // the other weight branches (FP64, no fused limiter)
 template <int ORDER>
-Sweep5Fn pick_mode(int mode, int lay, bool pair, bool copy)
+Sweep5Fn pick_mode(int mode, int lay, bool copy)
 {
     switch (mode) {
-    case MODE_TENO: return pick5<ORDER, MODE_TENO, double, false>(lay, pair, copy);
-    case MODE_WENOZ: return pick5<ORDER, MODE_WENOZ, double, false>(lay, pair, copy);
-    default: return pick5<ORDER, MODE_MAPPED, double, false>(lay, pair, copy);
+    case MODE_TENO: return pick5<ORDER, MODE_TENO, double, false>(lay, copy);
+    case MODE_WENOZ: return pick5<ORDER, MODE_WENOZ, double, false>(lay, copy);
+    default: return pick5<ORDER, MODE_MAPPED, double, false>(lay, copy);
     }
 }
 }  // namespace
@@ -728,11 +708,10 @@ hipError_t launch_sweep5(TileArgs a, int order, int mode, int lay, int dtype, in
     if (a.threads != kThreads5 || 2 * a.jn > kConsumers || a.ldj > 0xffff) return hipErrorInvalidConfiguration;
     if (mode != MODE_JS && (dtype != DT_F64 || mp)) return hipErrorInvalidConfiguration;
     const size_t lds = 2 * static_cast<size_t>(kNP) * a.ldj * sizeof(double);
-    const bool copy = a.copy_halo != 0, pair = a.pair_b != 0;
-    const Sweep5Fn kern = mode != MODE_JS ? (order == 7 ? pick_mode<7>(mode, lay, pair, copy)
-                                                        : pick_mode<5>(mode, lay, pair, copy))
-                          : order == 7    ? pick5<7>(dtype, lay, pair, copy, mp != 0)
-                                          : pick5<5>(dtype, lay, pair, copy, mp != 0);
+    const bool copy = a.copy_halo != 0;
+    const Sweep5Fn kern = mode != MODE_JS ? (order == 7 ? pick_mode<7>(mode, lay, copy) : pick_mode<5>(mode, lay, copy))
+                          : order == 7    ? pick5<7>(dtype, lay, copy, mp != 0)
+                                          : pick5<5>(dtype, lay, copy, mp != 0);
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
