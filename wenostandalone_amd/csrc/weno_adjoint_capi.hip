@@ -5,6 +5,7 @@
 
 namespace wm {
 hipError_t launch_vjp(TileArgs a, int order, double* grad, hipStream_t st);
+hipError_t launch_vjp5_tiles(TileArgs a, double* grad, hipStream_t st);
 int vjp_max_threads();
 int vjp_max_slots();
 int vjp_max_np();
@@ -52,6 +53,12 @@ extern "C" int weno_reconstruct_vjp(const weno_grid* g, int order, int dir, int 
     a.chunk = d.ns;
     a.chunks = 1;
     a.eps = g->weno_eps;
+    // WENO5 rows of up to 420 inputs: whole-row tiles with the
+    // shared-reciprocal adjoint (vjp5_tile_kernel)
+    if (order == 5) {
+        const hipError_t e = launch_vjp5_tiles(a, static_cast<double*>(o), stream());
+        if (e != hipErrorNotSupported) return finish(e, "reconstruct_vjp");
+    }
     const int R = order / 2, W = order;
     const int jn = a.j_end - a.j_beg + 1;
     const int nJ = jn + 2 * R;

@@ -96,6 +96,69 @@ __device__ __forceinline__ void tile_commit(const TileArgs& a, int nJ, int ld, i
     }
 }
 
+// Read-add-write of tile t of the array at `base` (the input's layout),
+// shared by `parts` waves: wave `part` takes the slots part, part + parts,
+// ... of the producer's slot mapping (PER of them, covering SLOTS), so the
+// accesses stay as coalesced as the producer's loads.  STORE = false loads
+// the elements into r; STORE = true adds b[q * ld + J] to r, stores the
+// element back and clears b[q * ld + J] for the next tile.  Only the tile's
+// own elements are stored (not the gaps between contiguous rows), so tiles
+// of one launch never write the same element.
+template <int SLOTS, int PER, bool STORE>
+__device__ __forceinline__ void tile_part(const TileArgs& a, void* base, int lo, int nJ, int ld, int t, int part,
+                                          int parts, double (&r)[PER], double* b)
+{
+    int l = threadIdx.x & 63;
+    asm volatile("" : "+v"(l));
+    const int np = a.np, G = 64 / np;
+    const int sq = l % np, J0 = l / np;
+    int iv, pa0, pb, cnt;
+    plane_tile(a, t, iv, pa0, pb, cnt);
+    const unsigned long long bm =
+        reinterpret_cast<unsigned long long>(base) +
+        8ull * (a.src_off + pa0 * a.s_a + pb * a.s_b + iv * a.s_i + static_cast<long long>(lo) * a.s_j);
+    using GD = __attribute__((address_space(1))) double;
+    if (a.s_j == 1) {
+        const int sa = static_cast<int>(a.s_a);
+        const float inv_sa = 1.0f / static_cast<float>(sa);
+        const int span = (cnt - 1) * sa + nJ;
+#pragma unroll
+        for (int m = 0; m < PER; ++m) {
+            const int s = part + parts * m;
+            const int e = l + 64 * s;
+            // element e of the span -> row q = e / s_a (exact in FP32, e < 2^11)
+            const int q = static_cast<int>((static_cast<float>(e) + 0.5f) * inv_sa), J = e - q * sa;
+            if (s < SLOTS && e < span && J < nJ) {
+                GD* const p = reinterpret_cast<GD*>(bm + 8u * static_cast<unsigned>(e));
+                if constexpr (STORE) {
+                    double* const g = b + q * ld + J;
+                    *p = r[m] + *g;
+                    *g = 0.0;
+                } else {
+                    r[m] = *p;
+                }
+            }
+        }
+    } else {
+        const unsigned off = static_cast<unsigned>(8 * (sq + J0 * a.s_j));
+        const unsigned long long stepb = 8ull * G * a.s_j;
+#pragma unroll
+        for (int m = 0; m < PER; ++m) {
+            const int s = part + parts * m;
+            if (s < SLOTS && sq < cnt && J0 + G * s < nJ) {
+                GD* const p = reinterpret_cast<GD*>(bm + stepb * s + off);
+                if constexpr (STORE) {
+                    double* const g = b + sq * ld + J0 + G * s;
+                    *p = r[m] + *g;
+                    *g = 0.0;
+                } else {
+                    r[m] = *p;
+                }
+            }
+        }
+    }
+}
+
 // Lane i receives lane i+1's value (DPP wave_shl:1) / lane i-1's value
 // (DPP wave_shr:1); the lanes at the wave's ends receive 0.
 __device__ __forceinline__ double from_next_lane(double x)

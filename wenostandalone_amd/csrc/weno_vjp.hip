@@ -14,6 +14,10 @@
 // stencils cover it; the tile's gradient leaves through the same coalesced
 // slot mapping the forward uses for loads (read-modify-write of gv).
 #include "weno_device.h"
+#include "weno_cells.h"
+#include "weno_tiles.h"
+
+#include <algorithm>
 
 namespace wm {
 
@@ -526,6 +530,422 @@ hipError_t launch_vjp(TileArgs a, int order, double* grad, hipStream_t st)
         hipLaunchKernelGGL(vjp_kernel<5>, dim3(bpc), dim3(a.threads), lds, st, a, grad);
     else
         hipLaunchKernelGGL(vjp_kernel<7>, dim3(bpc), dim3(a.threads), lds, st, a, grad);
+    return hipGetLastError();
+}
+
+// ---- WENO5-JS on whole-row tiles ------------------------------------------
+//
+// The adjoint of cell5's shared-reciprocal form.  Forward, with
+// D_i = v(i+1) - v(i) over the window v0..v4: smoothness b_r (b_0 on D3, D2;
+// b_1 on D2, D1; b_2 on D1, D0), m_r = prod_{s != r} b_s, u_r = m_r^2,
+// g^X_r = e^X_r D + f^X_r D' (d^X_r times candidate r's deviation from v2),
+// oX = v2 + N_X / S_X with N_X = sum_r u_r g^X_r, S_X = sum_r d^X_r u_r
+// (X = L, R), both sides normalised by one reciprocal of S_L S_R.
+// Backward, for cotangents gX of oX, with A_X = gX / S_X and
+// q_X = N_X / S_X = oX - v2:
+//   d/dg^X_r = A_X u_r,
+//   d/du_r   = A_L (g^L_r - q_L d^L_r) + A_R (g^R_r - q_R d^R_r) =: mu_r,
+//   d/db_0   = 2 (m_1 mu_1 b_2 + m_2 mu_2 b_1), and cyclically,
+// and the differences collect both paths (the candidate rows and the
+// smoothness quadratic forms).  About 140 FP64 operations per face pair
+// and one reciprocal.  Returns false when the shared product leaves the
+// normal range or a result is not finite; the lane then takes
+// vjp_cell5_slow.
+__device__ __forceinline__ bool vjp_cell5(double v0, double v1, double v2, double v3, double v4, const Coef5& k,
+                                          double eps, double gl, double gr, double (&c)[5])
+{
+    const double D3 = v4 - v3, D2 = v3 - v2, D1 = v2 - v1, D0 = v1 - v0;
+    const double S3 = D3 * D3, X3 = D3 * D2, S2 = D2 * D2, X2 = D2 * D1, S1 = D1 * D1, X1 = D1 * D0, S0 = D0 * D0;
+    const double b0 = fma(k.b[0], S3, fma(k.b[1], X3, fma(k.b[2], S2, eps)));
+    const double b1 = fma(k.b[3], S2, fma(k.b[4], X2, fma(k.b[5], S1, eps)));
+    const double b2 = fma(k.b[6], S1, fma(k.b[7], X1, fma(k.b[8], S0, eps)));
+    const double m0 = b1 * b2, m1 = b0 * b2, m2 = b0 * b1;
+    const double u0 = m0 * m0, u1 = m1 * m1, u2 = m2 * m2;
+    const double gL0 = fma(k.eL[0], D3, k.fL[0] * D2), gL1 = fma(k.eL[1], D2, k.fL[1] * D1),
+                 gL2 = fma(k.eL[2], D1, k.fL[2] * D0);
+    const double gR0 = fma(k.eR[0], D3, k.fR[0] * D2), gR1 = fma(k.eR[1], D2, k.fR[1] * D1),
+                 gR2 = fma(k.eR[2], D1, k.fR[2] * D0);
+    const double NL = fma(u0, gL0, fma(u1, gL1, u2 * gL2));
+    const double NR = fma(u0, gR0, fma(u1, gR1, u2 * gR2));
+    const double SL = fma(k.dL[0], u0, fma(k.dL[1], u1, k.dL[2] * u2));
+    const double SR = fma(k.dR[0], u0, fma(k.dR[1], u1, k.dR[2] * u2));
+    const double T = SL * SR;
+    double r = __builtin_amdgcn_rcp(T);
+#pragma unroll
+    for (int it = 0; it < 2; ++it) r = fma(r, fma(-T, r, 1.0), r);
+    const double iL = SR * r, iR = SL * r;   // 1 / S_L, 1 / S_R
+    const double qL = NL * iL, qR = NR * iR;
+    const double AL = gl * iL, AR = gr * iR;
+    const double AL2 = AL + AL, AR2 = AR + AR;
+    // 2 mu_r, then m_r * 2 mu_r
+    const double n0 = m0 * fma(AL2, fma(-qL, k.dL[0], gL0), AR2 * fma(-qR, k.dR[0], gR0));
+    const double n1 = m1 * fma(AL2, fma(-qL, k.dL[1], gL1), AR2 * fma(-qR, k.dR[1], gR1));
+    const double n2 = m2 * fma(AL2, fma(-qL, k.dL[2], gL2), AR2 * fma(-qR, k.dR[2], gR2));
+    // d/db_r
+    const double E0 = fma(n1, b2, n2 * b1), E1 = fma(n0, b2, n2 * b0), E2 = fma(n0, b1, n1 * b0);
+    // d/dg^X_r
+    const double aL0 = AL * u0, aL1 = AL * u1, aL2 = AL * u2;
+    const double aR0 = AR * u0, aR1 = AR * u1, aR2 = AR * u2;
+    const double T3 = D3 + D3, T2 = D2 + D2, T1 = D1 + D1, T0 = D0 + D0;
+    double gD3 = fma(aL0, k.eL[0], aR0 * k.eR[0]);
+    gD3 = fma(E0, fma(k.b[0], T3, k.b[1] * D2), gD3);
+    double gD2 = fma(aL0, k.fL[0], fma(aR0, k.fR[0], fma(aL1, k.eL[1], aR1 * k.eR[1])));
+    gD2 = fma(E0, fma(k.b[2], T2, k.b[1] * D3), gD2);
+    gD2 = fma(E1, fma(k.b[3], T2, k.b[4] * D1), gD2);
+    double gD1 = fma(aL1, k.fL[1], fma(aR1, k.fR[1], fma(aL2, k.eL[2], aR2 * k.eR[2])));
+    gD1 = fma(E1, fma(k.b[5], T1, k.b[4] * D2), gD1);
+    gD1 = fma(E2, fma(k.b[6], T1, k.b[7] * D0), gD1);
+    double gD0 = fma(aL2, k.fL[2], aR2 * k.fR[2]);
+    gD0 = fma(E2, fma(k.b[8], T0, k.b[7] * D1), gD0);
+    c[0] = -gD0;
+    c[1] = gD0 - gD1;
+    c[2] = (gD1 - gD2) + (gl + gr);
+    c[3] = gD2 - gD3;
+    c[4] = gD3;
+    // finite: class bits 3..8 (negative normal .. positive normal)
+    const bool fin = __builtin_amdgcn_class((gD0 + gD1) + (gD2 + gD3), 0x1f8);
+    return (static_cast<int>(pos_normal(T)) & static_cast<int>(pos_normal(r)) & static_cast<int>(fin)) != 0;
+}
+
+// The same contributions in the per-stencil form with IEEE divisions
+// (alpha_r = d_r / b_r^2, w_r = alpha_r / sum alpha, candidate deviation
+// g_r / d_r): d oX / d g^X_r = 1 / (b_r^2 sum alpha), d oX / d b_r =
+// -2 w_r (g_r / d_r - q_X) / b_r.  For lanes whose shared product leaves the
+// normal range; carries NaN and Inf as the forward does.  Always inlined (an
+// out-of-line call would keep the caller's coefficient registers in scratch).
+__device__ __forceinline__ void vjp_cell5_slow(double v0, double v1, double v2, double v3, double v4, const Coef5& k,
+                                               double eps, double gl, double gr, double (&c)[5])
+{
+    const double D3 = v4 - v3, D2 = v3 - v2, D1 = v2 - v1, D0 = v1 - v0;
+    const double b0 = fma(k.b[0], D3 * D3, fma(k.b[1], D3 * D2, fma(k.b[2], D2 * D2, eps)));
+    const double b1 = fma(k.b[3], D2 * D2, fma(k.b[4], D2 * D1, fma(k.b[5], D1 * D1, eps)));
+    const double b2 = fma(k.b[6], D1 * D1, fma(k.b[7], D1 * D0, fma(k.b[8], D0 * D0, eps)));
+    const double y0 = 1.0 / (b0 * b0), y1 = 1.0 / (b1 * b1), y2 = 1.0 / (b2 * b2);
+    double E0 = 0.0, E1 = 0.0, E2 = 0.0;
+    // one side: coefficients by value, so that nothing takes the address of
+    // the caller's coefficient registers
+    auto side = [&](double gx, double d0, double d1, double d2, double g0, double g1, double g2, double& a0,
+                    double& a1, double& a2) {
+        const double al0 = d0 * y0, al1 = d1 * y1, al2 = d2 * y2;
+        const double S = al0 + al1 + al2;
+        const double e0 = g0 / d0, e1 = g1 / d1, e2 = g2 / d2;   // candidate deviations from v2
+        const double qx = (al0 * e0 + al1 * e1 + al2 * e2) / S;
+        a0 = gx * y0 / S;
+        a1 = gx * y1 / S;
+        a2 = gx * y2 / S;
+        E0 -= 2.0 * gx * (al0 / S) * (e0 - qx) / b0;
+        E1 -= 2.0 * gx * (al1 / S) * (e1 - qx) / b1;
+        E2 -= 2.0 * gx * (al2 / S) * (e2 - qx) / b2;
+    };
+    double ga[2][3];
+    side(gl, k.dL[0], k.dL[1], k.dL[2], fma(k.eL[0], D3, k.fL[0] * D2), fma(k.eL[1], D2, k.fL[1] * D1),
+         fma(k.eL[2], D1, k.fL[2] * D0), ga[0][0], ga[0][1], ga[0][2]);
+    side(gr, k.dR[0], k.dR[1], k.dR[2], fma(k.eR[0], D3, k.fR[0] * D2), fma(k.eR[1], D2, k.fR[1] * D1),
+         fma(k.eR[2], D1, k.fR[2] * D0), ga[1][0], ga[1][1], ga[1][2]);
+    double gD3 = ga[0][0] * k.eL[0] + ga[1][0] * k.eR[0];
+    double gD2 = ga[0][0] * k.fL[0] + ga[1][0] * k.fR[0] + ga[0][1] * k.eL[1] + ga[1][1] * k.eR[1];
+    double gD1 = ga[0][1] * k.fL[1] + ga[1][1] * k.fR[1] + ga[0][2] * k.eL[2] + ga[1][2] * k.eR[2];
+    double gD0 = ga[0][2] * k.fL[2] + ga[1][2] * k.fR[2];
+    gD3 += E0 * (2.0 * k.b[0] * D3 + k.b[1] * D2);
+    gD2 += E0 * (k.b[1] * D3 + 2.0 * k.b[2] * D2) + E1 * (2.0 * k.b[3] * D2 + k.b[4] * D1);
+    gD1 += E1 * (k.b[4] * D2 + 2.0 * k.b[5] * D1) + E2 * (2.0 * k.b[6] * D1 + k.b[7] * D0);
+    gD0 += E2 * (k.b[7] * D1 + 2.0 * k.b[8] * D0);
+    c[0] = -gD0;
+    c[1] = gD0 - gD1;
+    c[2] = (gD1 - gD2) + (gl + gr);
+    c[3] = gD2 - gD3;
+    c[4] = gD3;
+}
+
+namespace {
+
+constexpr int kTileSlots = 28;      // producer registers per array (doubles per lane)
+constexpr int kTileMaxNP = 8;       // pencils per tile
+constexpr int kTileConsumers = 7;   // consumer waves (at most)
+constexpr int kSegment = 60;        // inputs owned per consumer wave
+constexpr int kMinConsumers = 4;    // consumer waves at least (they share the gradient write-back)
+static_assert(64 * kTileSlots <= 2048, "span element indices must stay exact in FP32");
+
+struct Vjp5Args {
+    TileArgs a;
+    double* grad;
+    int lo;     // input J of a tile row's first element (the core's first face - 2)
+    int nJ;     // inputs per row: the core and two on either side
+    int rows;   // rows laid end to end per pass
+};
+
+// One workgroup = up to 7 consumer waves + 1 producer wave, persistent over
+// tiles of np pencils (of one plane) x whole rows of inputs J = lo ..
+// lo+nJ-1, i.e. every input the core's faces read.  LDS: two input tiles
+// and two gradient tiles, [pencil][J] with an odd row stride.  One barrier
+// per tile.  The consumer's 27 coefficients and the adjoint cell take about
+// 150 VGPRs (at 128 the coefficients spill and are reloaded every pass):
+// 2 waves per SIMD, one workgroup per CU.
+//   producer : loads tile k+1's input and commits it to LDS (it never
+//              stores to global memory).
+//   consumers: the inputs of R rows, end to end, in segments of 60 per
+//              wave: wave w, lane i takes input position 60 w + i - 2 and
+//              the face there (the two inputs at either end of a row have
+//              no face).  Each lane forms its face's five contributions;
+//              lanes 2..61 own their input and sum the contributions of
+//              the faces two either side by four lane shifts, then write
+//              the sum to the gradient tile.  No atomics, the same order of
+//              additions in every run.  Lanes 0, 1, 62 and 63 repeat the
+//              neighbouring segments' faces.  The cotangents of the next
+//              pass (or of the next tile's first pass) are loaded one pass
+//              ahead.  In tile k's interval the consumer waves also add
+//              tile k-1's gradient tile to gv, in the producer's coalesced
+//              slot mapping split over the waves: the gv loads are issued
+//              before tile k's passes, the stores after them.
+__global__ void __attribute__((amdgpu_flat_work_group_size(64, 64 * (kTileConsumers + 1)), amdgpu_waves_per_eu(2)))
+vjp5_tile_kernel(const Vjp5Args va)
+{
+    const TileArgs& a = va.a;
+    extern __shared__ __align__(16) double lds[];
+    const int np = a.np, nJ = va.nJ, ld = a.ldj, lo = va.lo;
+    double* const vb0 = lds;
+    double* const vb1 = lds + np * ld;
+    double* const gb0 = lds + 2 * np * ld;
+    double* const gb1 = lds + 3 * np * ld;
+    const int tid = threadIdx.x;
+    const int nwave = blockDim.x >> 6;
+    const bool producer = __builtin_amdgcn_readfirstlane(tid >> 6) == nwave - 1;
+
+    // tiles t_first, t_first + tstep, ... < t_end; strided rows share input
+    // lines with the neighbouring tiles: one contiguous range per XCD
+    const int ntile = a.tiles_per_var * a.nv;
+    int t_first = blockIdx.x, t_end = ntile, tstep = gridDim.x;
+    if (a.s_j != 1 && (gridDim.x & 7) == 0) {
+        const int xcd = blockIdx.x & 7, rank = blockIdx.x >> 3;
+        const int per = ntile >> 3, rem = ntile & 7;
+        t_first = xcd * per + min(xcd, rem) + rank;
+        t_end = (xcd + 1) * per + min(xcd + 1, rem);
+        tstep = gridDim.x >> 3;
+    }
+    if (t_first >= t_end) return;
+
+    for (int e = tid; e < 2 * np * ld; e += blockDim.x) gb0[e] = 0.0;
+
+    if (producer) {
+        double r[kTileSlots];
+        tile_commit(a, nJ, ld, tile_issue(a, lo, nJ, t_first, r), r, vb0);
+        __syncthreads();
+        int k = 0;
+        for (int t = t_first; t < t_end; t += tstep, ++k) {
+            const int tn = t + tstep;
+            if (tn < t_end) tile_commit(a, nJ, ld, tile_issue(a, lo, nJ, tn, r), r, (k & 1) ? vb0 : vb1);
+            __syncthreads();
+        }
+        return;
+    }
+
+    const int i = tid & 63, cw = tid >> 6, nc = nwave - 1;
+    const int R = va.rows, npos = R * nJ;
+    const int pos = kSegment * cw + i - 2;
+    const int pc = min(max(pos, 0), npos - 1);
+    const int rr = pc / nJ, ii = pc - rr * nJ;
+    const bool inrange = pos >= 0 && pos < npos;
+    const bool own = inrange && i >= 2 && i < kSegment + 2;
+    const bool face = inrange && ii >= 2 && ii < nJ - 2;
+    const int j = lo - a.J0f + ii;   // output index of the face at this input
+    Coef5 k5;
+    {
+        int jc = j - 3;
+        jc = jc < 0 ? 0 : (jc > a.c.nc - 1 ? a.c.nc - 1 : jc);
+        load_coef5(a.c, jc, k5);
+    }
+    const int col = min(max(ii, 2), nJ - 3);
+    const long long o_a = a.a_is_k ? a.o_k : a.o_l, o_b = a.a_is_k ? a.o_l : a.o_k;
+    const double eps = a.eps;
+    // cotangents through buffer loads at 32-bit byte offsets (the face
+    // arrays are below 4 GiB, checked by the launcher); offsets past the end
+    // read 0, so lanes without a face in a pass load nothing and compute
+    // zero contributions
+    const int fbytes = static_cast<int>(static_cast<unsigned long long>(a.o_i) * a.nv * sizeof(double));
+    const __amdgpu_buffer_rsrc_t rgL = __builtin_amdgcn_make_buffer_rsrc(a.outL, 0, fbytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rgR = __builtin_amdgcn_make_buffer_rsrc(a.outR, 0, fbytes, 0x00020000);
+    const unsigned pass_step = static_cast<unsigned>(R * o_a * sizeof(double));
+    // byte offset of this lane's face in row rr of tile t (first pass), and the tile's pencil count
+    auto cot_off = [&](int t, int& cnt) {
+        int iv, pa0, pb;
+        plane_tile(a, t, iv, pa0, pb, cnt);
+        return static_cast<unsigned>((pa0 * o_a + pb * o_b + iv * a.o_i + j + rr * o_a) * sizeof(double));
+    };
+    auto cot = [&](__amdgpu_buffer_rsrc_t rs, unsigned off, bool valid) {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, valid ? off : 0xffffffffu, 0, 0));
+    };
+    // this lane's gradient slots (wave cw takes slots cw, cw + nc, ... of the
+    // producer's mapping), the same in every tile: pencil q in the high 16
+    // bits and LDS index q * ld + J in the low 16 (-1: none), and the
+    // element offset from the tile's first element
+    constexpr int PER = (kTileSlots + kMinConsumers - 1) / kMinConsumers;   // gradient slots per consumer lane
+    int sl_lds[PER];
+    unsigned sl_off[PER];
+    {
+        const int G = 64 / np, sq = i % np, J0 = i / np;
+#pragma unroll
+        for (int m = 0; m < PER; ++m) {
+            const int s = cw + nc * m;
+            int q, J;
+            unsigned off;
+            if (a.s_j == 1) {
+                const int e = i + 64 * s, sa = static_cast<int>(a.s_a);
+                q = e / sa;
+                J = e - q * sa;
+                off = static_cast<unsigned>(e);
+            } else {
+                q = sq;
+                J = J0 + G * s;
+                off = static_cast<unsigned>(sq + J * a.s_j);
+            }
+            const bool valid = s < kTileSlots && J < nJ && q < np;
+            sl_lds[m] = valid ? (q << 16) | (q * ld + J) : -1;
+            sl_off[m] = off;
+        }
+    }
+    double gv[PER];
+    auto grad_base = [&](int t, int& cnt) {
+        int iv, pa0, pb;
+        plane_tile(a, t, iv, pa0, pb, cnt);
+        return va.grad + (a.src_off + pa0 * a.s_a + pb * a.s_b + iv * a.s_i + static_cast<long long>(lo) * a.s_j);
+    };
+    using GD = __attribute__((address_space(1))) double;
+    int cnt;
+    unsigned off = cot_off(t_first, cnt);
+    double gl = cot(rgL, off, face && rr < cnt), gr = cot(rgR, off, face && rr < cnt);
+    __syncthreads();
+    int k = 0, tp = -1;
+    for (int t = t_first; t < t_end; t += tstep, ++k) {
+        const double* const cur = (k & 1) ? vb1 : vb0;
+        double* const gt = (k & 1) ? gb1 : gb0;
+        double* const gprev = (k & 1) ? gb0 : gb1;
+        // gv over tile k-1's rows, in flight during tile k's passes
+        int cntp = 0;
+        GD* const bp = reinterpret_cast<GD*>(reinterpret_cast<unsigned long long>(tp >= 0 ? grad_base(tp, cntp) : va.grad));
+        if (tp >= 0) {
+#pragma unroll
+            for (int m = 0; m < PER; ++m)
+                if (sl_lds[m] >= 0 && (sl_lds[m] >> 16) < cntp) gv[m] = bp[sl_off[m]];
+        }
+        const int tn = t + tstep;
+        int cntn = 0;
+        const unsigned offn = tn < t_end ? cot_off(tn, cntn) : 0u;
+        for (int q0 = 0; q0 < cnt; q0 += R) {
+            const int q = q0 + rr;
+            const bool live = face && q < cnt;
+            // cotangents one pass ahead: this tile's next pass, or the next tile's first
+            double gln, grn;
+            if (q0 + R < cnt) {
+                gln = cot(rgL, off + pass_step, face && q + R < cnt);
+                grn = cot(rgR, off + pass_step, face && q + R < cnt);
+            } else {
+                gln = cot(rgL, offn, face && rr < cntn);
+                grn = cot(rgR, offn, face && rr < cntn);
+            }
+            const double* const rw = cur + min(q, cnt - 1) * ld + col;
+            const double v0 = rw[-2], v1 = rw[-1], v2 = rw[0], v3 = rw[1], v4 = rw[2];
+            double c[5];
+            const bool ok = vjp_cell5(v0, v1, v2, v3, v4, k5, eps, gl, gr, c);
+            // Lanes without a face have zero cotangents, so a finite cell gives
+            // exactly zero contributions; a wave-wide (scalar) test keeps the
+            // rare other case a branch: the slow form for faces, zeros for
+            // the rest.
+            if (__builtin_amdgcn_ballot_w64(!ok) != 0) {
+                double x0 = v0, x1 = v1, x2 = v2, x3 = v3, x4 = v4;
+                asm volatile("" : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4));
+                if (!ok) {
+                    if (live) {
+                        vjp_cell5_slow(x0, x1, x2, x3, x4, k5, eps, gl, gr, c);
+                    } else {
+#pragma unroll
+                        for (int o = 0; o < 5; ++o) c[o] = 0.0;
+                    }
+                }
+            }
+            // input position p collects c[2] of its own face, c[3], c[4] of
+            // the faces p-1, p-2 and c[1], c[0] of the faces p+1, p+2
+            const double A = c[3] + from_prev_lane(c[4]);
+            const double B = c[1] + from_next_lane(c[0]);
+            const double sum = c[2] + (from_prev_lane(A) + from_next_lane(B));
+            if (own && q < cnt) gt[q * ld + ii] = sum;
+            gl = gln;
+            gr = grn;
+            off += pass_step;
+        }
+        if (tp >= 0) {
+#pragma unroll
+            for (int m = 0; m < PER; ++m) {
+                if (sl_lds[m] >= 0 && (sl_lds[m] >> 16) < cntp) {
+                    double* const g = gprev + (sl_lds[m] & 0xffff);
+                    bp[sl_off[m]] = gv[m] + *g;
+                    *g = 0.0;
+                }
+            }
+        }
+        tp = t;
+        off = offn;
+        cnt = cntn;
+        __syncthreads();
+    }
+    // the last tile's gradient, after its barrier
+    {
+        int cntp = 0;
+        GD* const bp = reinterpret_cast<GD*>(reinterpret_cast<unsigned long long>(grad_base(tp, cntp)));
+        double* const gprev = (k & 1) ? gb0 : gb1;
+#pragma unroll
+        for (int m = 0; m < PER; ++m) {
+            if (sl_lds[m] >= 0 && (sl_lds[m] >> 16) < cntp) {
+                double* const g = gprev + (sl_lds[m] & 0xffff);
+                bp[sl_off[m]] = bp[sl_off[m]] + *g;
+            }
+        }
+    }
+}
+
+}  // namespace
+
+// Rows of at most kTileConsumers * kSegment inputs (core + 4) whose tiles
+// fit the producer's registers; returns hipErrorNotSupported otherwise (the
+// caller then takes vjp_kernel).
+hipError_t launch_vjp5_tiles(TileArgs a, double* grad, hipStream_t st)
+{
+    Vjp5Args va;
+    va.lo = a.j_beg - 2 + a.J0f;
+    va.nJ = (a.j_end - a.j_beg + 1) + 4;
+    va.grad = grad;
+    if (va.nJ > kTileConsumers * kSegment || (a.s_j != 1 && a.s_a != 1)) return hipErrorNotSupported;
+    // 32-bit byte offsets into the face arrays, 32-bit element offsets of a
+    // tile's gradient slots
+    if (static_cast<unsigned long long>(a.o_i) * a.nv * sizeof(double) > 0xffffffffull ||
+        static_cast<unsigned long long>(va.nJ + 1) * static_cast<unsigned long long>(a.s_j) + 64 > 0xffffffffull)
+        return hipErrorNotSupported;
+    a.np = tile_pencils(va.nJ, a.s_a, a.s_j == 1, kTileSlots, kTileMaxNP);
+    if (a.np < 1) return hipErrorNotSupported;
+    va.rows = std::max(1, std::min(a.np, kTileConsumers * kSegment / va.nJ));
+    const int nseg = (va.rows * va.nJ + kSegment - 1) / kSegment;
+    // waves past the last segment own no input and share the gradient write-back
+    a.threads = 64 * (std::max(nseg, kMinConsumers) + 1);
+    a.ldj = va.nJ | 1;
+    a.chunks = (a.na + a.np - 1) / a.np;   // tiles per plane
+    a.tiles_per_var = a.chunks * a.nb;
+    va.a = a;
+    const size_t lds = 4 * static_cast<size_t>(a.np) * a.ldj * sizeof(double);
+    const void* kern = reinterpret_cast<const void*>(vjp5_tile_kernel);
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+        if (e != hipSuccess) return e;
+    }
+    int occ = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, a.threads, lds) != hipSuccess || occ < 1) occ = 1;
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    const long long ntile = static_cast<long long>(a.tiles_per_var) * a.nv;
+    long long grid = std::min<long long>(ntile, 1LL * cus * occ);
+    if (a.s_j != 1 && grid >= 8) grid &= ~7LL;   // whole XCD ranges
+    hipLaunchKernelGGL(vjp5_tile_kernel, dim3(static_cast<unsigned>(std::max<long long>(1, grid))), dim3(a.threads),
+                       lds, st, va);
     return hipGetLastError();
 }
 
