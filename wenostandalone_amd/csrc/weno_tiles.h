@@ -12,8 +12,9 @@
 //     64 consecutive elements per instruction; element e of the span is LDS
 //     row e / s_a, column e % s_a;
 //   strided rows (y, z sweeps of the field, s_a = 1): the tile's pencils are
-//     consecutive in memory; lane l holds pencil l % np of rows
-//     J = l / np + (64 / np) s.
+//     consecutive in memory; with G = 64 / np rows per instruction, lane
+//     l < np G holds pencil l % np of rows J = l / np + G s (the other lanes
+//     idle when np does not divide 64).
 #pragma once
 
 #include "weno_internal.h"
@@ -62,7 +63,7 @@ __device__ __forceinline__ int tile_issue(const TileArgs& a, int lo, int nJ, int
         const unsigned long long stepb = 8ull * G * a.s_j;
 #pragma unroll
         for (int s = 0; s < SLOTS; ++s)
-            if (sq < cnt && J0 + G * s < nJ) r[s] = *reinterpret_cast<GlobalIn<double>*>(bm + stepb * s + off);
+            if (sq < cnt && J0 < G && J0 + G * s < nJ) r[s] = *reinterpret_cast<GlobalIn<double>*>(bm + stepb * s + off);
     }
     return cnt;
 }
@@ -92,7 +93,7 @@ __device__ __forceinline__ void tile_commit(const TileArgs& a, int nJ, int ld, i
     } else {
 #pragma unroll
         for (int s = 0; s < SLOTS; ++s)
-            if (sq < cnt && J0 + G * s < nJ) b[sq * ld + J0 + G * s] = r[s];
+            if (sq < cnt && J0 < G && J0 + G * s < nJ) b[sq * ld + J0 + G * s] = r[s];
     }
 }
 
@@ -145,7 +146,7 @@ __device__ __forceinline__ void tile_part(const TileArgs& a, void* base, int lo,
 #pragma unroll
         for (int m = 0; m < PER; ++m) {
             const int s = part + parts * m;
-            if (s < SLOTS && sq < cnt && J0 + G * s < nJ) {
+            if (s < SLOTS && sq < cnt && J0 < G && J0 + G * s < nJ) {
                 GD* const p = reinterpret_cast<GD*>(bm + stepb * s + off);
                 if constexpr (STORE) {
                     double* const g = b + sq * ld + J0 + G * s;

@@ -547,18 +547,22 @@ hipError_t launch_vjp(TileArgs a, int order, double* grad, hipStream_t st)
 //   d/du_r   = A_L (g^L_r - q_L d^L_r) + A_R (g^R_r - q_R d^R_r) =: mu_r,
 //   d/db_0   = 2 (m_1 mu_1 b_2 + m_2 mu_2 b_1), and cyclically,
 // and the differences collect both paths (the candidate rows and the
-// smoothness quadratic forms).  About 140 FP64 operations per face pair
+// smoothness quadratic forms).  About 125 FP64 operations per face pair
 // and one reciprocal.  Returns false when the shared product leaves the
 // normal range or a result is not finite; the lane then takes
 // vjp_cell5_slow.
 __device__ __forceinline__ bool vjp_cell5(double v0, double v1, double v2, double v3, double v4, const Coef5& k,
-                                          double eps, double gl, double gr, double (&c)[5])
+                                          const double (&blo2)[3], double eps, double gl, double gr, double (&c)[5])
 {
     const double D3 = v4 - v3, D2 = v3 - v2, D1 = v2 - v1, D0 = v1 - v0;
-    const double S3 = D3 * D3, X3 = D3 * D2, S2 = D2 * D2, X2 = D2 * D1, S1 = D1 * D1, X1 = D1 * D0, S0 = D0 * D0;
-    const double b0 = fma(k.b[0], S3, fma(k.b[1], X3, fma(k.b[2], S2, eps)));
-    const double b1 = fma(k.b[3], S2, fma(k.b[4], X2, fma(k.b[5], S1, eps)));
-    const double b2 = fma(k.b[6], S1, fma(k.b[7], X1, fma(k.b[8], S0, eps)));
+    // b_r = B_hi hi^2 + B_x hi lo + B_lo lo^2 + eps over (hi, lo) = (D3, D2),
+    // (D2, D1), (D1, D0); with p_r = B_hi hi + B_x lo, b_r = hi p_r + B_lo lo^2
+    // + eps, db_r/dhi = p_r + B_hi hi, db_r/dlo = B_x hi + 2 B_lo lo
+    const double p0 = fma(k.b[0], D3, k.b[1] * D2), p1 = fma(k.b[3], D2, k.b[4] * D1),
+                 p2 = fma(k.b[6], D1, k.b[7] * D0);
+    const double b0 = fma(D3, p0, fma(k.b[2], D2 * D2, eps));
+    const double b1 = fma(D2, p1, fma(k.b[5], D1 * D1, eps));
+    const double b2 = fma(D1, p2, fma(k.b[8], D0 * D0, eps));
     const double m0 = b1 * b2, m1 = b0 * b2, m2 = b0 * b1;
     const double u0 = m0 * m0, u1 = m1 * m1, u2 = m2 * m2;
     const double gL0 = fma(k.eL[0], D3, k.fL[0] * D2), gL1 = fma(k.eL[1], D2, k.fL[1] * D1),
@@ -586,17 +590,16 @@ __device__ __forceinline__ bool vjp_cell5(double v0, double v1, double v2, doubl
     // d/dg^X_r
     const double aL0 = AL * u0, aL1 = AL * u1, aL2 = AL * u2;
     const double aR0 = AR * u0, aR1 = AR * u1, aR2 = AR * u2;
-    const double T3 = D3 + D3, T2 = D2 + D2, T1 = D1 + D1, T0 = D0 + D0;
     double gD3 = fma(aL0, k.eL[0], aR0 * k.eR[0]);
-    gD3 = fma(E0, fma(k.b[0], T3, k.b[1] * D2), gD3);
+    gD3 = fma(E0, fma(k.b[0], D3, p0), gD3);
     double gD2 = fma(aL0, k.fL[0], fma(aR0, k.fR[0], fma(aL1, k.eL[1], aR1 * k.eR[1])));
-    gD2 = fma(E0, fma(k.b[2], T2, k.b[1] * D3), gD2);
-    gD2 = fma(E1, fma(k.b[3], T2, k.b[4] * D1), gD2);
+    gD2 = fma(E0, fma(blo2[0], D2, k.b[1] * D3), gD2);
+    gD2 = fma(E1, fma(k.b[3], D2, p1), gD2);
     double gD1 = fma(aL1, k.fL[1], fma(aR1, k.fR[1], fma(aL2, k.eL[2], aR2 * k.eR[2])));
-    gD1 = fma(E1, fma(k.b[5], T1, k.b[4] * D2), gD1);
-    gD1 = fma(E2, fma(k.b[6], T1, k.b[7] * D0), gD1);
+    gD1 = fma(E1, fma(blo2[1], D1, k.b[4] * D2), gD1);
+    gD1 = fma(E2, fma(k.b[6], D1, p2), gD1);
     double gD0 = fma(aL2, k.fL[2], aR2 * k.fR[2]);
-    gD0 = fma(E2, fma(k.b[8], T0, k.b[7] * D1), gD0);
+    gD0 = fma(E2, fma(blo2[2], D0, k.b[7] * D1), gD0);
     c[0] = -gD0;
     c[1] = gD0 - gD1;
     c[2] = (gD1 - gD2) + (gl + gr);
@@ -660,7 +663,7 @@ __device__ __forceinline__ void vjp_cell5_slow(double v0, double v1, double v2, 
 namespace {
 
 constexpr int kTileSlots = 28;      // producer registers per array (doubles per lane)
-constexpr int kTileMaxNP = 8;       // pencils per tile
+constexpr int kTileMaxNP = 16;      // pencils per tile
 constexpr int kTileConsumers = 7;   // consumer waves (at most)
 constexpr int kSegment = 60;        // inputs owned per consumer wave
 constexpr int kMinConsumers = 4;    // consumer waves at least (they share the gradient write-back)
@@ -696,7 +699,10 @@ struct Vjp5Args {
 //              ahead.  In tile k's interval the consumer waves also add
 //              tile k-1's gradient tile to gv, in the producer's coalesced
 //              slot mapping split over the waves: the gv loads are issued
-//              before tile k's passes, the stores after them.
+//              before tile k's passes, the stores after them.  Every
+//              element of a gradient tile is written in its tile's interval
+//              (each input has one owner lane), so the tiles are never
+//              cleared.
 __global__ void __attribute__((amdgpu_flat_work_group_size(64, 64 * (kTileConsumers + 1)), amdgpu_waves_per_eu(2)))
 vjp5_tile_kernel(const Vjp5Args va)
 {
@@ -723,8 +729,6 @@ vjp5_tile_kernel(const Vjp5Args va)
         tstep = gridDim.x >> 3;
     }
     if (t_first >= t_end) return;
-
-    for (int e = tid; e < 2 * np * ld; e += blockDim.x) gb0[e] = 0.0;
 
     if (producer) {
         double r[kTileSlots];
@@ -754,6 +758,7 @@ vjp5_tile_kernel(const Vjp5Args va)
         jc = jc < 0 ? 0 : (jc > a.c.nc - 1 ? a.c.nc - 1 : jc);
         load_coef5(a.c, jc, k5);
     }
+    const double blo2[3] = {k5.b[2] + k5.b[2], k5.b[5] + k5.b[5], k5.b[8] + k5.b[8]};   // 2 B_lo
     const int col = min(max(ii, 2), nJ - 3);
     const long long o_a = a.a_is_k ? a.o_k : a.o_l, o_b = a.a_is_k ? a.o_l : a.o_k;
     const double eps = a.eps;
@@ -776,8 +781,9 @@ vjp5_tile_kernel(const Vjp5Args va)
     };
     // this lane's gradient slots (wave cw takes slots cw, cw + nc, ... of the
     // producer's mapping), the same in every tile: pencil q in the high 16
-    // bits and LDS index q * ld + J in the low 16 (-1: none), and the
-    // element offset from the tile's first element
+    // bits and LDS index q * ld + J in the low 16 (no slot: pencil 0x7fff,
+    // which no tile has, and index 0), and the element offset from the
+    // tile's first element
     constexpr int PER = (kTileSlots + kMinConsumers - 1) / kMinConsumers;   // gradient slots per consumer lane
     int sl_lds[PER];
     unsigned sl_off[PER];
@@ -794,12 +800,12 @@ vjp5_tile_kernel(const Vjp5Args va)
                 J = e - q * sa;
                 off = static_cast<unsigned>(e);
             } else {
-                q = sq;
+                q = J0 < G ? sq : np;   // lanes past np G idle
                 J = J0 + G * s;
                 off = static_cast<unsigned>(sq + J * a.s_j);
             }
             const bool valid = s < kTileSlots && J < nJ && q < np;
-            sl_lds[m] = valid ? (q << 16) | (q * ld + J) : -1;
+            sl_lds[m] = valid ? (q << 16) | (q * ld + J) : 0x7fff << 16;
             sl_off[m] = off;
         }
     }
@@ -825,7 +831,7 @@ vjp5_tile_kernel(const Vjp5Args va)
         if (tp >= 0) {
 #pragma unroll
             for (int m = 0; m < PER; ++m)
-                if (sl_lds[m] >= 0 && (sl_lds[m] >> 16) < cntp) gv[m] = bp[sl_off[m]];
+                if ((sl_lds[m] >> 16) < cntp) gv[m] = bp[sl_off[m]];
         }
         const int tn = t + tstep;
         int cntn = 0;
@@ -845,7 +851,7 @@ vjp5_tile_kernel(const Vjp5Args va)
             const double* const rw = cur + min(q, cnt - 1) * ld + col;
             const double v0 = rw[-2], v1 = rw[-1], v2 = rw[0], v3 = rw[1], v4 = rw[2];
             double c[5];
-            const bool ok = vjp_cell5(v0, v1, v2, v3, v4, k5, eps, gl, gr, c);
+            const bool ok = vjp_cell5(v0, v1, v2, v3, v4, k5, blo2, eps, gl, gr, c);
             // Lanes without a face have zero cotangents, so a finite cell gives
             // exactly zero contributions; a wave-wide (scalar) test keeps the
             // rare other case a branch: the slow form for faces, zeros for
@@ -873,14 +879,14 @@ vjp5_tile_kernel(const Vjp5Args va)
             off += pass_step;
         }
         if (tp >= 0) {
+            // every (pencil < cnt, J < nJ) of a gradient tile has one owner
+            // lane that writes it in its tile's interval: nothing to clear
+            double gs[PER];
 #pragma unroll
-            for (int m = 0; m < PER; ++m) {
-                if (sl_lds[m] >= 0 && (sl_lds[m] >> 16) < cntp) {
-                    double* const g = gprev + (sl_lds[m] & 0xffff);
-                    bp[sl_off[m]] = gv[m] + *g;
-                    *g = 0.0;
-                }
-            }
+            for (int m = 0; m < PER; ++m) gs[m] = gprev[sl_lds[m] & 0xffff];
+#pragma unroll
+            for (int m = 0; m < PER; ++m)
+                if ((sl_lds[m] >> 16) < cntp) bp[sl_off[m]] = gv[m] + gs[m];
         }
         tp = t;
         off = offn;
@@ -894,10 +900,7 @@ vjp5_tile_kernel(const Vjp5Args va)
         double* const gprev = (k & 1) ? gb0 : gb1;
 #pragma unroll
         for (int m = 0; m < PER; ++m) {
-            if (sl_lds[m] >= 0 && (sl_lds[m] >> 16) < cntp) {
-                double* const g = gprev + (sl_lds[m] & 0xffff);
-                bp[sl_off[m]] = bp[sl_off[m]] + *g;
-            }
+            if ((sl_lds[m] >> 16) < cntp) bp[sl_off[m]] = bp[sl_off[m]] + gprev[sl_lds[m] & 0xffff];
         }
     }
 }
@@ -919,9 +922,28 @@ hipError_t launch_vjp5_tiles(TileArgs a, double* grad, hipStream_t st)
     if (static_cast<unsigned long long>(a.o_i) * a.nv * sizeof(double) > 0xffffffffull ||
         static_cast<unsigned long long>(va.nJ + 1) * static_cast<unsigned long long>(a.s_j) + 64 > 0xffffffffull)
         return hipErrorNotSupported;
-    a.np = tile_pencils(va.nJ, a.s_a, a.s_j == 1, kTileSlots, kTileMaxNP);
+    // pencils per tile and rows per pass: the tile's pencils take
+    // ceil(np / R) passes of R rows over max(nseg, kMinConsumers) waves;
+    // take the shape with the most face lanes per lane-pass (then the
+    // larger tile) among those whose tile fits the producer's registers
+    const int nf = va.nJ - 4;
+    double best = 0.0;
+    a.np = 0;
+    for (int np = 1; np <= kTileMaxNP; ++np) {
+        const bool rows = a.s_j == 1;
+        const long long need = rows ? ((np - 1) * a.s_a + va.nJ + 63) / 64
+                                    : (va.nJ + 64 / np - 1) / (64 / np);
+        if (need > kTileSlots) continue;
+        const int R = std::max(1, std::min(np, kTileConsumers * kSegment / va.nJ));
+        const int waves = std::max((R * va.nJ + kSegment - 1) / kSegment, kMinConsumers);
+        const double use = double(np) * nf / (double((np + R - 1) / R) * 64.0 * waves);
+        if (use > best + 1e-9 || (use > best - 1e-9 && np > a.np)) {
+            best = use;
+            a.np = np;
+            va.rows = R;
+        }
+    }
     if (a.np < 1) return hipErrorNotSupported;
-    va.rows = std::max(1, std::min(a.np, kTileConsumers * kSegment / va.nJ));
     const int nseg = (va.rows * va.nJ + kSegment - 1) / kSegment;
     // waves past the last segment own no input and share the gradient write-back
     a.threads = 64 * (std::max(nseg, kMinConsumers) + 1);
