@@ -200,3 +200,59 @@ def test_vjp_longest_supported_sweep(order, m):
     rc = wm.lib.weno_reconstruct_vjp(C.byref(g1), order, d, wm.SRC_RESHAPED, rs.vp, C.byref(coef),
                                      dgL.vp, dgR.vp, gv.vp)
     assert rc == -1 and "core points" in wm._lib.last_error()
+
+
+def _vjp_pair(order, d, f, eps, m, n, p, v, b, seed):
+    g = wm.make_grid(m, n, p, v=v, buff=b, eps=eps)
+    og = oracle.grid(m, n, p, v=v, buff=b, eps=eps)
+    ns, nk, nl = fields.dir_extents(m, n, p, b, d)
+    cs = fields.harness_coefficients(ns - 6, fields.DIR_OFFSETS[d])
+    devs = [DeviceArray.from_host(c) for c in cs]
+    coef = wm.Coeffs(*[x.ptr for x in devs])
+    gL, gR = cotangents((ns, nk, nl, v), seed=seed)
+    v_rs = oracle.reshape(og, d, f)
+    ref = oracle.vjp(og, order, d, v_rs, cs, gL, gR)
+    dgL, dgR = DeviceArray.from_host(gL), DeviceArray.from_host(gR)
+    rs = DeviceArray.from_host(v_rs)
+    gv = DeviceArray(v_rs.shape)
+    gv.fill(0.0)
+    wm.check(wm.lib.weno_reconstruct_vjp(C.byref(g), order, d, wm.SRC_RESHAPED, rs.vp, C.byref(coef),
+                                         dgL.vp, dgR.vp, gv.vp))
+    return gv.download(), ref
+
+
+@pytest.mark.parametrize("order", [5, 7])
+@pytest.mark.parametrize("case", ["scale1e22", "scale1e-30", "spikes", "nan"])
+def test_vjp_out_of_range_products(order, case):
+    """Inputs whose shared weight product leaves the normal range, so lanes
+    take the per-stencil IEEE form: a field scaled by 1e22 (overflow) or by
+    1e-30 with eps 1e-100 (underflow) in every lane; isolated 1e22 cells in
+    an O(1) field (fast and slow lanes in one wavefront: the WENO5 tile kernel
+    redoes such a tile's passes); NaN cells (NaN in exactly the gradient
+    entries the oracle makes NaN, the rest to the usual bound).  WENO
+    reconstruction is homogeneous of degree one in the state, so the
+    gradient stays O(1) at every scale: normwise 1e-10 against the oracle."""
+    m, n, p, v, b = 37, 11, 9, 2, 6
+    rng = np.random.default_rng(400 + order)
+    f = fields.seeded_field(m, n, p, v, b, seed=401)
+    eps = 1e-16
+    if case == "scale1e22":
+        f = np.asfortranarray(f * 1e22)
+    elif case == "scale1e-30":
+        f = np.asfortranarray(f * 1e-30)
+        eps = 1e-100
+    elif case == "spikes":
+        for _ in range(12):
+            f[tuple(int(rng.integers(3, s - 3)) for s in f.shape[:3]) + (0,)] = 1e22
+    else:
+        for _ in range(3):
+            f[tuple(int(rng.integers(3, s - 3)) for s in f.shape[:3]) + (1,)] = np.nan
+    for d in range(3):
+        print(f"vjp out of range: order {order} case {case} dir {d}", flush=True)
+        got, ref = _vjp_pair(order, d, f, eps, m, n, p, v, b, seed=410 + d)
+        nan = np.isnan(ref)
+        assert np.array_equal(np.isnan(got), nan), (d, int(nan.sum()), int(np.isnan(got).sum()))
+        if case == "nan":
+            assert nan.any(), d
+        fin = ~nan
+        assert norm_err(got[fin], ref[fin]) <= 1e-10, d

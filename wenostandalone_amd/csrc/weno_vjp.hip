@@ -548,9 +548,11 @@ hipError_t launch_vjp(TileArgs a, int order, double* grad, hipStream_t st)
 //   d/db_0   = 2 (m_1 mu_1 b_2 + m_2 mu_2 b_1), and cyclically,
 // and the differences collect both paths (the candidate rows and the
 // smoothness quadratic forms).  About 125 FP64 operations per face pair
-// and one reciprocal.  Returns false when the shared product leaves the
-// normal range or a result is not finite; the lane then takes
-// vjp_cell5_slow.
+// and one reciprocal.  Returns false when the shared product T = S_L S_R or
+// its reciprocal leaves the normal range; the lane then takes
+// vjp_cell5_slow.  With T normal nothing in the backward overflows for
+// finite cotangents: S_L and S_R weight the same u_r by the linear weights,
+// so each is within a small factor of sqrt(T), above 1e-154.
 __device__ __forceinline__ bool vjp_cell5(double v0, double v1, double v2, double v3, double v4, const Coef5& k,
                                           const double (&blo2)[3], double eps, double gl, double gr, double (&c)[5])
 {
@@ -605,9 +607,7 @@ __device__ __forceinline__ bool vjp_cell5(double v0, double v1, double v2, doubl
     c[2] = (gD1 - gD2) + (gl + gr);
     c[3] = gD2 - gD3;
     c[4] = gD3;
-    // finite: class bits 3..8 (negative normal .. positive normal)
-    const bool fin = __builtin_amdgcn_class((gD0 + gD1) + (gD2 + gD3), 0x1f8);
-    return (static_cast<int>(pos_normal(T)) & static_cast<int>(pos_normal(r)) & static_cast<int>(fin)) != 0;
+    return (static_cast<int>(pos_normal(T)) & static_cast<int>(pos_normal(r))) != 0;
 }
 
 // The same contributions in the per-stencil form with IEEE divisions
@@ -661,6 +661,18 @@ __device__ __forceinline__ void vjp_cell5_slow(double v0, double v1, double v2, 
 }
 
 namespace {
+
+// Input position p (lane p of the wave) collects c[2] of its own face, c[3],
+// c[4] of the faces p-1, p-2 and c[1], c[0] of the faces p+1, p+2: four
+// lane shifts (the wave's two end lanes on either side get partial sums).
+// Call it with every lane of the wave active: a lane shift reads 0 from a
+// lane that is switched off.
+__device__ __forceinline__ double lane_sum(const double (&c)[5])
+{
+    const double A = c[3] + from_prev_lane(c[4]);
+    const double B = c[1] + from_next_lane(c[0]);
+    return c[2] + (from_prev_lane(A) + from_next_lane(B));
+}
 
 constexpr int kTileSlots = 28;      // producer registers per array (doubles per lane)
 constexpr int kTileMaxNP = 16;      // pencils per tile
@@ -836,9 +848,10 @@ vjp5_tile_kernel(const Vjp5Args va)
         const int tn = t + tstep;
         int cntn = 0;
         const unsigned offn = tn < t_end ? cot_off(tn, cntn) : 0u;
+        const unsigned off0 = off;
+        bool allok = true;
         for (int q0 = 0; q0 < cnt; q0 += R) {
             const int q = q0 + rr;
-            const bool live = face && q < cnt;
             // cotangents one pass ahead: this tile's next pass, or the next tile's first
             double gln, grn;
             if (q0 + R < cnt) {
@@ -851,32 +864,42 @@ vjp5_tile_kernel(const Vjp5Args va)
             const double* const rw = cur + min(q, cnt - 1) * ld + col;
             const double v0 = rw[-2], v1 = rw[-1], v2 = rw[0], v3 = rw[1], v4 = rw[2];
             double c[5];
-            const bool ok = vjp_cell5(v0, v1, v2, v3, v4, k5, blo2, eps, gl, gr, c);
-            // Lanes without a face have zero cotangents, so a finite cell gives
-            // exactly zero contributions; a wave-wide (scalar) test keeps the
-            // rare other case a branch: the slow form for faces, zeros for
-            // the rest.
-            if (__builtin_amdgcn_ballot_w64(!ok) != 0) {
-                double x0 = v0, x1 = v1, x2 = v2, x3 = v3, x4 = v4;
+            // lanes without a face have zero cotangents, so a cell in range
+            // gives them exactly zero contributions
+            allok &= vjp_cell5(v0, v1, v2, v3, v4, k5, blo2, eps, gl, gr, c);
+            // the lane shifts run with every lane of the wave active
+            const double sum = lane_sum(c);
+            if (own && q < cnt) gt[q * ld + ii] = sum;
+            gl = gln;
+            gr = grn;
+            off += pass_step;
+        }
+        // Rare: a lane's shared product left the normal range.  The wave
+        // redoes the tile's passes (it owns the positions it wrote): the
+        // slow form for faces, zeros for the lanes without one.
+        if (__builtin_amdgcn_ballot_w64(!allok) != 0) {
+            unsigned offr = off0;
+            for (int q0 = 0; q0 < cnt; q0 += R) {
+                const int q = q0 + rr;
+                const bool live = face && q < cnt;
+                const double glr = cot(rgL, offr, live), grr = cot(rgR, offr, live);
+                const double* const rw = cur + min(q, cnt - 1) * ld + col;
+                double x0 = rw[-2], x1 = rw[-1], x2 = rw[0], x3 = rw[1], x4 = rw[2];
+                double c[5];
+                const bool ok = vjp_cell5(x0, x1, x2, x3, x4, k5, blo2, eps, glr, grr, c);
                 asm volatile("" : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4));
                 if (!ok) {
                     if (live) {
-                        vjp_cell5_slow(x0, x1, x2, x3, x4, k5, eps, gl, gr, c);
+                        vjp_cell5_slow(x0, x1, x2, x3, x4, k5, eps, glr, grr, c);
                     } else {
 #pragma unroll
                         for (int o = 0; o < 5; ++o) c[o] = 0.0;
                     }
                 }
+                const double sum = lane_sum(c);
+                if (own && q < cnt) gt[q * ld + ii] = sum;
+                offr += pass_step;
             }
-            // input position p collects c[2] of its own face, c[3], c[4] of
-            // the faces p-1, p-2 and c[1], c[0] of the faces p+1, p+2
-            const double A = c[3] + from_prev_lane(c[4]);
-            const double B = c[1] + from_next_lane(c[0]);
-            const double sum = c[2] + (from_prev_lane(A) + from_next_lane(B));
-            if (own && q < cnt) gt[q * ld + ii] = sum;
-            gl = gln;
-            gr = grn;
-            off += pass_step;
         }
         if (tp >= 0) {
             // every (pencil < cnt, J < nJ) of a gradient tile has one owner
