@@ -11,7 +11,8 @@ gfx950 code object and reads each kernel's metadata (llvm-readelf --notes):
     coefficient struct's address makes the compiler keep the coefficients in
     a 240-byte scratch frame written at every launch (63 MB of extra HBM
     writes per 200^3 x 8 sweep, 5 % of its time);
-  * the fused consumer-step kernels of the JS weights carry no scratch.
+  * the fused consumer-step kernels of the JS weights and the WENO5 adjoint's
+    tile kernel carry no scratch.
 """
 import os
 import re
@@ -97,3 +98,11 @@ def test_js_consumer_step_kernels_have_no_scratch(kernels):
     js = {n: v for n, v in kernels.items() if "fused_flux_kernelILi5ELi1E" in n or "fused_flux_kernelILi7ELi1E" in n}
     assert js
     assert all(s == 0 for s, _ in js.values()), js
+
+
+def test_adjoint_tile_kernel_has_no_scratch(kernels):
+    # the WENO5 adjoint's lanes hold 27 coefficients for the whole launch: at
+    # 128 VGPRs they spilled and were reloaded every pass
+    k = {n: v for n, v in kernels.items() if "vjp5_tile_kernel" in n}
+    assert len(k) == 1, k
+    assert all(s == 0 for s, _ in k.values()), k
