@@ -678,7 +678,7 @@ constexpr int kTileSlots = 28;      // producer registers per array (doubles per
 constexpr int kTileMaxNP = 16;      // pencils per tile
 constexpr int kTileConsumers = 7;   // consumer waves (at most)
 constexpr int kSegment = 60;        // inputs owned per consumer wave
-constexpr int kMinConsumers = 4;    // consumer waves at least (they share the gradient write-back)
+constexpr int kMinConsumers = 7;    // consumer waves (those past the last segment share the gradient write-back)
 static_assert(64 * kTileSlots <= 2048, "span element indices must stay exact in FP32");
 
 struct Vjp5Args {
