@@ -669,7 +669,6 @@ int weno_reshape(const weno_grid* g, int dir, int dtype, const void* v_vf, void*
     if ((rc = ensure_init())) return rc;
     TileArgs a;
     std::memset(&a, 0, sizeof a);
-    Launch l{};
     const Dims d = dims_of(g, dir);
     const void* s;
     void* o;
@@ -686,10 +685,7 @@ int weno_reshape(const weno_grid* g, int dir, int dtype, const void* v_vf, void*
     a.j_end = 0;
     a.copy_halo = 1;
     a.sides = 1;
-    l.op = OP_RESHAPE;
-    l.dtype = dtype;
-    plan_tiles(a, l, 0, 0);
-    return finish(launch_tile(a, l, stream()), "reshape");
+    return finish(launch_permute(a, dtype, stream()), "reshape");
 }
 
 int weno_order1_copy(const weno_grid* g, int dir, int dtype, const void* v_vf, void* vL, void* vR)
@@ -699,7 +695,6 @@ int weno_order1_copy(const weno_grid* g, int dir, int dtype, const void* v_vf, v
         return rc;
     TileArgs a;
     std::memset(&a, 0, sizeof a);
-    Launch l{};
     const Dims d = dims_of(g, dir);
     const void* s;
     void *oL, *oR;
@@ -717,10 +712,7 @@ int weno_order1_copy(const weno_grid* g, int dir, int dtype, const void* v_vf, v
     a.j_end = 0;
     a.copy_halo = 1;
     a.sides = 3;
-    l.op = OP_COPY;
-    l.dtype = dtype;
-    plan_tiles(a, l, 0, 0);
-    return finish(launch_tile(a, l, stream()), "order1_copy");
+    return finish(launch_permute(a, dtype, stream()), "order1_copy");
 }
 
 int weno_reconstruct(const weno_grid* g, int order, int dir, int src_kind, int dtype, const void* src,

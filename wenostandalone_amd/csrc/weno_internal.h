@@ -77,6 +77,10 @@ int max_threads(int op, int order);
 // Launches one tile-kernel sweep; returns hipSuccess or an error.
 hipError_t launch_tile(const TileArgs& a, const Launch& l, hipStream_t st);
 
+// Order-1 copy and reshape (rows of the field permuted along the sweep; no
+// tile planning): streaming rows for x sweeps, LDS-transposed tiles for y/z.
+hipError_t launch_permute(const TileArgs& a, int dtype, hipStream_t st);
+
 // WENO5-JS producer/consumer kernel: planning (returns the layout, or -1 if
 // the launch stays on the general kernel) and launch.
 int plan_sweep5(TileArgs& a, Launch& l, const void* src);

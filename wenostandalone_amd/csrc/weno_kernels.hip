@@ -1,6 +1,7 @@
-// HIP kernels for gfx950: the sweep kernel behind reshape, order-1 copy,
-// WENO3/5/7 (JS and TENO), the fused or standalone monotonicity-preserving
-// limiter, in FP64, FP32 and BF16-state storage.  WENO5-JS launches that
+// HIP kernels for gfx950: the general sweep kernel behind WENO3/5/7 (every
+// weight branch), the fused or standalone monotonicity-preserving limiter,
+// in FP64, FP32 and BF16-state storage (the reshape and the order-1 copy run
+// on weno_permute.hip).  WENO5-JS launches that
 // write both sides over the full reconstructed range (rows of at least 96
 // outputs, aligned arrays) run on the producer/consumer kernel of
 // weno_sweep5.hip instead (launch_tile hands them over).
@@ -391,15 +392,8 @@ hipError_t launch_tile(const TileArgs& a, const Launch& l, hipStream_t st)
     if (a.np > kMaxNP || a.slots > kMaxSlots || a.threads > max_threads(l.op, l.order) || a.threads < 64 ||
         a.ldj > 0xffff)
         return hipErrorInvalidConfiguration;
+    // the reshape and the order-1 copy run on launch_permute
     switch (l.op) {
-    case OP_RESHAPE:
-        if (l.dtype == DT_F64) return go<OP_RESHAPE, 5, MODE_JS, false, double, double>(a, l, st);
-        if (l.dtype == DT_F32) return go<OP_RESHAPE, 5, MODE_JS, false, float, float>(a, l, st);
-        return hipErrorInvalidValue;
-    case OP_COPY:
-        if (l.dtype == DT_F64) return go<OP_COPY, 5, MODE_JS, false, double, double>(a, l, st);
-        if (l.dtype == DT_F32) return go<OP_COPY, 5, MODE_JS, false, float, float>(a, l, st);
-        return go<OP_COPY, 5, MODE_JS, false, bf16_t, float>(a, l, st);
     case OP_MP:
         if (l.dtype == DT_F64) return go<OP_MP, 5, MODE_JS, true, double, double>(a, l, st);
         if (l.dtype == DT_F32) return go<OP_MP, 5, MODE_JS, true, float, float>(a, l, st);
