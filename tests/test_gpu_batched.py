@@ -1,10 +1,17 @@
 """Batched blocks in one persistent launch per direction, hipGraph capture of
 whole x/y/z sweeps, and the tile-shape autotuner.
 
-Every batched output is compared with the oracle sequence of its own block
-(1e-13 relative, bitwise for the order-1 halo); graph replays are compared
-bitwise with the same launches run eagerly, replayed several times with
-eager work and synchronisation in between."""
+Every batched output is compared with the oracle sequence of its own block,
+bitwise for the order-1 halo.  These blocks use random coefficients whose
+smoothness indicators are not positive definite, so a face can be a small
+difference of order-1 candidates: there the reference's own FP64 evaluation
+differs from an extended-precision evaluation of the same formula by 1.1e-13
+of the face value (block 5 of SPECS, x sweep) but by 5e-16 of the stencil's
+values.  The faces are therefore held to 1e-13 relative to the stencil's
+scale (rel_err_stencil); the config-3 blocks with the harness coefficients
+(tests/test_gpu_shapes.py) are held to 1e-13 of the face value.  Graph replays
+are compared bitwise with the same launches run eagerly, replayed several
+times with eager work and synchronisation in between."""
 import ctypes as C
 
 import numpy as np
@@ -14,7 +21,7 @@ import oracle
 import wenostandalone_amd as wm
 from wenostandalone_amd import fields
 from wenostandalone_amd.device import DeviceArray
-from tests.conftest import rel_err
+from tests.conftest import rel_err, rel_err_stencil, stencil_scale
 
 pytestmark = pytest.mark.gpu
 
@@ -69,6 +76,16 @@ class Blocks:
     def download(self):
         return [[(L.download(), R.download()) for L, R in fc] for fc in self.faces]
 
+    def scale(self, b, d):
+        """Largest |v| of each face's stencil (the limiter's window for WENO3)."""
+        m, n, p, v, order = self.specs[b]
+        og = oracle.grid(m, n, p, v=v, buff=6)
+        return stencil_scale(oracle.reshape(og, d, self.fields[b]), max(order // 2, 2))
+
+    def close(self, b, d, got, exp, tol=1e-13):
+        sc = self.scale(b, d)
+        return rel_err_stencil(got, exp, sc) <= tol
+
     def expected(self, b, d, flags):
         m, n, p, v, order = self.specs[b]
         og = oracle.grid(m, n, p, v=v, buff=6)
@@ -96,8 +113,8 @@ def test_batched_vs_oracle(flags):
         for d in range(3):
             eL, eR = bl.expected(b, d, flags)
             L, R = got[b][d]
-            assert rel_err(L, eL) <= 1e-13, (b, d)
-            assert rel_err(R, eR) <= 1e-13, (b, d)
+            assert bl.close(b, d, L, eL), (b, d)
+            assert bl.close(b, d, R, eR), (b, d)
 
 
 def test_batched_core_only_keeps_halo():
@@ -112,7 +129,8 @@ def test_batched_core_only_keeps_halo():
             assert np.all(L[:3] == 9.5) and np.all(R[-3:] == -9.5)
         eL, eR = bl.expected(b, 0, 0)
         s = slice(0, None) if order == 3 else slice(3, -3)
-        assert rel_err(L[s], eL[s]) <= 1e-13 and rel_err(R[s], eR[s]) <= 1e-13
+        sc = bl.scale(b, 0)[s]
+        assert rel_err_stencil(L[s], eL[s], sc) <= 1e-13 and rel_err_stencil(R[s], eR[s], sc) <= 1e-13
 
 
 def test_batched_graph_replays():
@@ -206,4 +224,4 @@ def test_batched_autotune():
     got = bl.download()
     for b in range(len(SPECS)):
         eL, eR = bl.expected(b, 1, wm.FUSE_COPY)
-        assert rel_err(got[b][1][0], eL) <= 1e-13 and rel_err(got[b][1][1], eR) <= 1e-13
+        assert bl.close(b, 1, got[b][1][0], eL) and bl.close(b, 1, got[b][1][1], eR)

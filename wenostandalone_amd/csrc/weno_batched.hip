@@ -10,11 +10,13 @@
 // and coefficients.  The counters are reset by a one-thread kernel (no
 // memset), so a whole x/y/z sweep can be captured in one hipGraph.
 #include "weno_device.h"
+#include "weno_cells.h"
 #include "weno_capi_internal.h"
 
 #include <algorithm>
 #include <cstring>
 #include <vector>
+#include <type_traits>
 
 namespace wm {
 
@@ -131,8 +133,12 @@ __device__ __forceinline__ void fill_tile(const TileArgs& a, const ItemGeom& g, 
 }
 
 // A run of consecutive items of one block (descriptor D), starting at `item`.
-// The block's coefficients are loaded once for the run: the candidate
-// polynomial coefficients into registers, beta and d into an LDS table
+// The block's coefficients are loaded once for the run.  JS weights (WENO5
+// and WENO7) take the sweep kernels' shared-reciprocal cells (cell5/cell7)
+// with every coefficient in registers (64-block batch 7.58 -> 6.49 ms per
+// x+y+z; the workgroup alternates load and compute phases, so the cells'
+// fewer instructions shorten it directly); the other branches keep the
+// candidate coefficients in registers and beta and d in an LDS table
 // [lane][entry] after the tile.  Per item: pencil addresses, tile, one
 // barrier, the claim of the next item (thread 0; its round trip overlaps the
 // reconstruction), reconstruction and stores, one barrier after which every
@@ -158,12 +164,19 @@ __device__ void batch_block(const BatchDesc& D, int item, double* tile, long lon
     const bool core = active && g.has_core && j >= g.c0 && j <= g.c1;
     const bool do_mp = MP && core && j >= a.mp_beg && j <= a.mp_end;
     LdsMutF64* const table = (LdsMutF64*)(tile + a.np * a.ldj);
+    constexpr bool CELL = MODE == MODE_JS && ORDER != 3;
+    std::conditional_t<ORDER == 7, Coef7, Coef5> kc;
     Cf<ORDER, double> f;
     if (core) {
         int jc = j - 3;
         jc = jc < 0 ? 0 : (jc > a.c.nc - 1 ? a.c.nc - 1 : jc);
-        load_cf(a.c, jc, f);
-        if (gq == 0) {   // the groups share the table (same j per lane index)
+        if constexpr (CELL) {
+            if constexpr (ORDER == 5) load_coef5(a.c, jc, kc);
+            else load_coef7(a.c, jc, kc);
+        } else {
+            load_cf(a.c, jc, f);
+        }
+        if (!CELL && gq == 0) {   // the groups share the table (same j per lane index)
             const GlbCoef *beta = gcoef(a.c.beta), *dL = gcoef(a.c.dL), *dR = gcoef(a.c.dR);
 #pragma unroll
             for (int t = 0; t < NB; ++t) {
@@ -203,7 +216,13 @@ __device__ void batch_block(const BatchDesc& D, int item, double* tile, long lon
                 for (int o = 0; o < W; ++o) v[o] = row[o - R];
                 double oL, oR;
                 const double* vc = v + (R - RW);
-                if constexpr (ORDER == 3)
+                if constexpr (CELL && ORDER == 5) {
+                    if (!cell5<2>(vc[0], vc[1], vc[2], vc[3], vc[4], kc, a.eps, oL, oR))
+                        cell5_slow_inl(vc[0], vc[1], vc[2], vc[3], vc[4], kc, a.eps, oL, oR);
+                } else if constexpr (CELL && ORDER == 7) {
+                    const double w[7] = {vc[0], vc[1], vc[2], vc[3], vc[4], vc[5], vc[6]};
+                    if (!cell7<2>(w, kc, a.eps, oL, oR)) cell7_slow_inl(w, kc, a.eps, oL, oR);
+                } else if constexpr (ORDER == 3)
                     weno3<MODE, double, false>(vc, f, B, a.eps, a.teno_ct, oL, oR);
                 else if constexpr (ORDER == 5)
                     weno5<MODE, double, false>(vc, f, B, a.eps, a.teno_ct, oL, oR);
