@@ -11,8 +11,9 @@ gfx950 code object and reads each kernel's metadata (llvm-readelf --notes):
     coefficient struct's address makes the compiler keep the coefficients in
     a 240-byte scratch frame written at every launch (63 MB of extra HBM
     writes per 200^3 x 8 sweep, 5 % of its time);
-  * the fused consumer-step kernels of the JS weights and the WENO5 adjoint's
-    tile kernel carry no scratch.
+  * the fused consumer-step kernels of the JS weights, the WENO5 adjoint's
+    tile kernel, the reshape / order-1 copy kernels and the batched kernel
+    carry no scratch.
 """
 import os
 import re
@@ -106,3 +107,16 @@ def test_adjoint_tile_kernel_has_no_scratch(kernels):
     k = {n: v for n, v in kernels.items() if "vjp5_tile_kernel" in n}
     assert len(k) == 1, k
     assert all(s == 0 for s, _ in k.values()), k
+
+
+def test_permute_and_batched_kernels_have_no_scratch(kernels):
+    # reshape / order-1 copy: pure data movement; batched: the JS cells keep
+    # every coefficient in registers (186-198 VGPRs, one 512-thread
+    # workgroup per CU)
+    perm = {n: v for n, v in kernels.items() if "permute_" in n}
+    # flat (3 storage types x 3 pair loads), rows (3), tiles (3 x paired or not)
+    assert len(perm) >= 18, sorted(perm)
+    assert all(s == 0 for s, _ in perm.values()), perm
+    batched = {n: v for n, v in kernels.items() if "14batched_kernel" in n}
+    assert len(batched) == 2, batched
+    assert all(s == 0 and r <= 256 for s, r in batched.values()), batched
