@@ -146,13 +146,16 @@ def test_vjp_central_differences_16cube(order):
 
 @pytest.mark.parametrize("order", [5, 7])
 @pytest.mark.parametrize("m,n,p,v,dirs", [
-    (47, 47, 47, 8, (0, 1, 2)),   # 54-point rows: 64-thread workgroups of one lane group
-    (243, 17, 17, 2, (0,)),       # 250-point rows: 256-thread workgroups, two per CU
-    (127, 59, 59, 4, (0,)),       # 135-point rows: 448-thread workgroups, ~6 tiles each
+    (47, 47, 47, 8, (0, 1, 2)),   # 54-point rows (WENO7: 64-thread workgroups of one lane group)
+    (243, 17, 17, 2, (0,)),       # 250-point rows (WENO7: 256-thread workgroups, two per CU)
+    (127, 59, 59, 4, (0,)),       # 135-point rows (WENO7: 448-thread workgroups, ~6 tiles each)
 ])
 def test_vjp_many_tiles_per_workgroup(order, m, n, p, v, dirs):
-    """Launches where every workgroup runs several tiles in turn (the WENO5
-    path loads tile t+1 while it reconstructs tile t), against the oracle."""
+    """Launches where every workgroup runs several tiles in turn, against the
+    oracle: WENO5 on the whole-row tile kernel (the producer wave loads tile
+    t+1 while the consumer waves work on tile t; 60-input segments per wave,
+    so these rows take one, four and three segments), WENO7 on the gather
+    kernel with the shapes noted per case."""
     b = 6
     g = wm.make_grid(m, n, p, v=v, buff=b)
     og = oracle.grid(m, n, p, v=v, buff=b)
