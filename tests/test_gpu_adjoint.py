@@ -153,9 +153,9 @@ def test_vjp_central_differences_16cube(order):
 def test_vjp_many_tiles_per_workgroup(order, m, n, p, v, dirs):
     """Launches where every workgroup runs several tiles in turn, against the
     oracle: WENO5 on the whole-row tile kernel (the producer wave loads tile
-    t+1 while the consumer waves work on tile t; 60-input segments per wave,
-    so these rows take one, four and three segments), WENO7 on the gather
-    kernel with the shapes noted per case."""
+    t+1 while the consumer waves work on tile t; each wave owns a segment of
+    60 input positions, so the longer rows span several waves), WENO7 on the
+    gather kernel with the shapes noted per case."""
     b = 6
     g = wm.make_grid(m, n, p, v=v, buff=b)
     og = oracle.grid(m, n, p, v=v, buff=b)
