@@ -6,16 +6,18 @@
 // * x sweeps (rows contiguous in v_vf): a flat streaming copy, each lane
 //   moving two adjacent elements of one output row (16-byte stores for FP64
 //   faces; one 16-byte load where the input pair is aligned, as for the
-//   reshape); the row and pencil of a pair come from multiply-high
-//   divisions by the row length and extents.  Odd row lengths and outputs
-//   that are not aligned to a pair take one element per lane, one wave per
-//   row at a time.
+//   reshape, or the aligned pair one element back plus the next lane's
+//   element, as for the copy's rows); the row and pencil of a pair come
+//   from multiply-high divisions by the row length and extents.  Odd row
+//   lengths and outputs that are not aligned to a pair take one element per
+//   lane, one wave per row at a time.
 // * y and z sweeps: tiles of 16 pencils that are consecutive in x (the
 //   field's unit-stride coordinate) by a chunk of the row along the sweep,
 //   transposed through LDS.  Loads: lane -> (pencil t & 15, row t >> 4), so a
 //   wave instruction reads 4 rows of 16 pencils (128 contiguous bytes per
 //   row for FP64); eight loads per lane are in flight before the first LDS
-//   store.  Stores: 64 consecutive outputs of one row per wave instruction.
+//   store.  Stores: 64 pairs of consecutive outputs of one row per wave
+//   instruction (1 KB for FP64; 64 single outputs where rows are odd).
 //   Neighbouring tiles share the input lines at their edges (a 16-pencil
 //   segment starts 24 bytes into a line), so tiles go to the XCDs in groups
 //   of 32 consecutive tiles and a shared line is fetched into one L2.
