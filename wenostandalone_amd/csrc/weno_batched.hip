@@ -29,7 +29,7 @@ struct BatchDesc {
 };
 
 constexpr int kBatchThreads = 512;
-constexpr int kBatchMaxNP = 64;
+constexpr int kBatchMaxNP = 256;
 constexpr int kBatchLdsDescs = 1024;
 
 // reconstruct the tile in LDS with order ORDER (after the tile barrier)

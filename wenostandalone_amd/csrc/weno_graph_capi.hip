@@ -112,7 +112,7 @@ int describe_block(const weno_block& B, int b, int dir, int flags, int threads, 
     int npg = std::max(1, np_target / G);
     npg = std::min(npg, std::max(1, batch_max_np() / G));
     auto lds_of = [&](int np) { return (size_t(np) * ldj + size_t(entries_of(B.order)) * jn) * sizeof(double); };
-    while (npg > 1 && lds_of(G * npg) > 96 * 1024) --npg;
+    while (npg > 1 && lds_of(G * npg) > 150 * 1024) --npg;
     a.groups = std::min(G, batch_max_np());
     a.npg = npg;
     a.np = a.groups * npg;
@@ -413,7 +413,7 @@ int weno_batched_autotune(const weno_block* blocks, int nblocks, int flags, int*
     float best_ms = 1e30f;
     BatchTune bt;
     for (int th : {256, 512})
-        for (int np : {8, 16, 32, 64}) {
+        for (int np : {8, 16, 32, 64, 128, 256}) {
             BatchPlan plans[3];
             bool ok = true;
             for (int d = 0; d < 3 && ok; ++d) ok = plan_batch(blocks, nblocks, d, flags, th, np, plans[d]) == WENO_OK;
